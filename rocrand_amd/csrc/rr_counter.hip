@@ -1,0 +1,262 @@
+// rr_counter.hip -- Philox4x32-10 and ThreeFry{2,4}x{32,64}-20 bulk generation on gfx950.
+//
+// Output order reproduces generate_philox (library/src/rng/philox4x32_10.hpp:106-214)
+// and generate_threefry (library/src/rng/threefry.hpp:86-198) under the DEFAULT
+// grid: vec index v (one engine block of DIM words, FULL outputs) starts at word
+// position base + DIM*v, where base = engine position + H, except that vec
+// indices that are multiples of the reference grid stride start at engine
+// position + C0 (the words the head actually consumed).  The reference's grid is
+// not this kernel's grid: any schedule that computes that map is equivalent.
+//
+// Schedule: grid-stride over vecs, each lane producing one vec per iteration so
+// a wavefront writes 64 consecutive vecs (1-2 KiB) with 16-byte stores.
+#include "rr_launch.hpp"
+
+namespace rr
+{
+namespace
+{
+
+template<class W, int DIM>
+__device__ __forceinline__ void interleave(const W* a, const W* b, uint32_t sub, W* o)
+{
+    if constexpr(DIM == 4)
+    {
+        switch(sub)
+        {
+            case 0: o[0] = a[0]; o[1] = a[1]; o[2] = a[2]; o[3] = a[3]; break;
+            case 1: o[0] = a[1]; o[1] = a[2]; o[2] = a[3]; o[3] = b[0]; break;
+            case 2: o[0] = a[2]; o[1] = a[3]; o[2] = b[0]; o[3] = b[1]; break;
+            default: o[0] = a[3]; o[1] = b[0]; o[2] = b[1]; o[3] = b[2]; break;
+        }
+    }
+    else
+    {
+        if(sub == 0)
+        {
+            o[0] = a[0];
+            o[1] = a[1];
+        }
+        else
+        {
+            o[0] = a[1];
+            o[1] = b[0];
+        }
+    }
+}
+
+// Sequential reader over the word stream (the engine's next()), for head/tail.
+template<ctr_kind K>
+struct seq_reader
+{
+    using T    = ctr_traits<K>;
+    using W    = typename T::word;
+    static constexpr int DIM = T::DIM;
+    W        ctr[DIM];
+    W        cur[DIM];
+    uint32_t sub;
+    __device__ seq_reader(const ctr_pos& p, uint64_t add_ctr, const ctr_key& key)
+    {
+#pragma unroll
+        for(int i = 0; i < DIM; i++)
+            ctr[i] = (W)p.c[i];
+        ctr_add<W, DIM>(ctr, add_ctr);
+        sub = p.sub;
+        T::block(ctr, key, cur);
+    }
+    __device__ W next(const ctr_key& key)
+    {
+        W r = cur[0];
+#pragma unroll
+        for(int i = 1; i < DIM; i++)
+            if((int)sub == i)
+                r = cur[i];
+        sub++;
+        if((int)sub == DIM)
+        {
+            sub = 0;
+            ctr_add<W, DIM>(ctr, 1);
+            T::block(ctr, key, cur);
+        }
+        return r;
+    }
+};
+
+template<int BYTES>
+__device__ __forceinline__ void store_vec(void* dst, const uint32_t* u)
+{
+    if constexpr(BYTES == 8)
+    {
+        *reinterpret_cast<uint2*>(dst) = make_uint2(u[0], u[1]);
+    }
+    else
+    {
+#pragma unroll
+        for(int j = 0; j < BYTES / 16; j++)
+            reinterpret_cast<uint4*>(dst)[j] = make_uint4(u[4 * j], u[4 * j + 1], u[4 * j + 2], u[4 * j + 3]);
+    }
+}
+
+template<ctr_kind K, class D>
+__global__ __launch_bounds__(256) void counter_kernel(counter_call c)
+{
+    using T                    = ctr_traits<K>;
+    using W                    = typename T::word;
+    using out_t                = typename D::out_t;
+    constexpr int DIM          = T::DIM;
+    constexpr int IN           = D::in_w;
+    constexpr int OUT          = D::out_w;
+    constexpr int OPT          = DIM / IN;
+    constexpr int FULL         = OPT * OUT;
+    constexpr int BYTES        = FULL * (int)sizeof(out_t);
+    static_assert(DIM % IN == 0, "input width");
+    static_assert(BYTES == 8 || BYTES == 16 || BYTES == 32, "vec size");
+
+    out_t* const  data     = static_cast<out_t*>(c.data);
+    out_t* const  vec_data = data + c.mis;
+    const size_t  stride   = (size_t)gridDim.x * blockDim.x;
+
+    for(size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < c.vec_n; v += stride)
+    {
+        const ctr_pos& b = (c.quirk && (v % CTR_REF_STRIDE) == 0) ? c.vec_base0 : c.vec_base;
+        W              ctr[DIM];
+#pragma unroll
+        for(int i = 0; i < DIM; i++)
+            ctr[i] = (W)b.c[i];
+        ctr_add<W, DIM>(ctr, v);
+        W w[DIM];
+        T::block(ctr, c.key, w);
+        W words[DIM];
+        if(b.sub != 0)
+        {
+            W w2[DIM];
+            ctr_add<W, DIM>(ctr, 1);
+            T::block(ctr, c.key, w2);
+            interleave<W, DIM>(w, w2, b.sub, words);
+        }
+        else
+        {
+#pragma unroll
+            for(int i = 0; i < DIM; i++)
+                words[i] = w[i];
+        }
+        out_t o[FULL];
+#pragma unroll
+        for(int s = 0; s < OPT; s++)
+            D::apply(words + s * IN, o + s * OUT, c.params);
+        uint32_t u[BYTES / 4];
+        __builtin_memcpy(u, o, BYTES);
+        store_vec<BYTES>(vec_data + v * FULL, u);
+    }
+
+    if(blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        out_t tmp[OUT];
+        if(c.head > 0)
+        {
+            // Head: consecutive words from the engine position (thread 0 of the reference grid).
+            seq_reader<K> r(c.pos, 0, c.key);
+            for(int s = 0; s < OPT; s++)
+            {
+                if((uint32_t)(s * OUT) >= c.head)
+                    break;
+                W in[IN];
+#pragma unroll
+                for(int i = 0; i < IN; i++)
+                    in[i] = r.next(c.key);
+                D::apply(in, tmp, c.params);
+                for(int o = 0; o < OUT; o++)
+                    if((uint32_t)(s * OUT + o) < c.head)
+                        data[s * OUT + o] = tmp[o];
+            }
+        }
+        if(c.tail > 0)
+        {
+            // Tail: the thread whose next vec index is vec_n continues from that vec's position.
+            const ctr_pos& b = (c.quirk && (c.vec_n % CTR_REF_STRIDE) == 0) ? c.vec_base0 : c.vec_base;
+            seq_reader<K>  r(b, c.vec_n, c.key);
+            for(int s = 0; s < OPT; s++)
+            {
+                if((uint32_t)(s * OUT) >= c.tail)
+                    break;
+                W in[IN];
+#pragma unroll
+                for(int i = 0; i < IN; i++)
+                    in[i] = r.next(c.key);
+                D::apply(in, tmp, c.params);
+                for(int o = 0; o < OUT; o++)
+                    if((uint32_t)(s * OUT + o) < c.tail)
+                        data[c.n - c.tail + s * OUT + o] = tmp[o];
+            }
+        }
+    }
+}
+
+template<ctr_kind K, class D>
+hipError_t launch_one(const counter_call& c, hipStream_t s)
+{
+    const size_t max_blocks = 8192;
+    size_t       blocks     = (c.vec_n + 255) / 256;
+    if(blocks > max_blocks)
+        blocks = max_blocks;
+    if(blocks == 0)
+        blocks = 1;
+    hipLaunchKernelGGL((counter_kernel<K, D>), dim3((unsigned)blocks), dim3(256), 0, s, c);
+    return hipGetLastError();
+}
+
+template<ctr_kind K>
+hipError_t dispatch_kind(out_kind o, bool one_word, const counter_call& c, hipStream_t s)
+{
+    (void)one_word;
+    constexpr word_family F = (K == ctr_kind::threefry2x64 || K == ctr_kind::threefry4x64) ? word_family::w64
+                                                                                            : word_family::w32;
+    switch(o)
+    {
+        case out_kind::u8: return launch_one<K, dist<F, out_kind::u8>>(c, s);
+        case out_kind::u16: return launch_one<K, dist<F, out_kind::u16>>(c, s);
+        case out_kind::u32: return launch_one<K, dist<F, out_kind::u32>>(c, s);
+        case out_kind::u64:
+            if constexpr(F == word_family::w64)
+                return launch_one<K, dist<F, out_kind::u64>>(c, s);
+            else
+                return hipErrorInvalidValue;
+        case out_kind::uniform_f32: return launch_one<K, dist<F, out_kind::uniform_f32>>(c, s);
+        case out_kind::uniform_f64: return launch_one<K, dist<F, out_kind::uniform_f64>>(c, s);
+        case out_kind::uniform_f16: return launch_one<K, dist<F, out_kind::uniform_f16>>(c, s);
+        case out_kind::normal_f32: return launch_one<K, dist<F, out_kind::normal_f32>>(c, s);
+        case out_kind::lognormal_f32: return launch_one<K, dist<F, out_kind::lognormal_f32>>(c, s);
+        case out_kind::normal_f16: return launch_one<K, dist<F, out_kind::normal_f16>>(c, s);
+        case out_kind::lognormal_f16: return launch_one<K, dist<F, out_kind::lognormal_f16>>(c, s);
+        case out_kind::poisson: return launch_one<K, dist<F, out_kind::poisson>>(c, s);
+        case out_kind::normal_f64:
+            // ThreeFry2x32 doubles use the one-word erfinv transform (threefry.hpp:441-449).
+            if constexpr(K == ctr_kind::threefry2x32)
+                return launch_one<K, dist<F, out_kind::normal_f64, true>>(c, s);
+            else
+                return launch_one<K, dist<F, out_kind::normal_f64>>(c, s);
+        case out_kind::lognormal_f64:
+            if constexpr(K == ctr_kind::threefry2x32)
+                return launch_one<K, dist<F, out_kind::lognormal_f64, true>>(c, s);
+            else
+                return launch_one<K, dist<F, out_kind::lognormal_f64>>(c, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+} // namespace
+
+hipError_t launch_counter(ctr_kind k, out_kind o, bool one_word, const counter_call& c, hipStream_t s)
+{
+    switch(k)
+    {
+        case ctr_kind::philox4x32: return dispatch_kind<ctr_kind::philox4x32>(o, one_word, c, s);
+        case ctr_kind::threefry2x32: return dispatch_kind<ctr_kind::threefry2x32>(o, one_word, c, s);
+        case ctr_kind::threefry2x64: return dispatch_kind<ctr_kind::threefry2x64>(o, one_word, c, s);
+        case ctr_kind::threefry4x32: return dispatch_kind<ctr_kind::threefry4x32>(o, one_word, c, s);
+        case ctr_kind::threefry4x64: return dispatch_kind<ctr_kind::threefry4x64>(o, one_word, c, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+} // namespace rr

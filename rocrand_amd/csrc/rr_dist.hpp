@@ -1,0 +1,569 @@
+// rr_dist.hpp -- output transforms (uniform / normal / log-normal / Poisson) on gfx950.
+//
+// Each functor maps IN_W engine words to OUT_W outputs, reproducing the
+// reference functors of library/src/rng/distribution/{uniform,normal,log_normal,
+// poisson}.hpp over the scalar math of rocrand_{uniform,normal,discrete}.h.
+//
+// Exactness: integer and uniform outputs are bit-exact (the scalings are exact
+// powers of two or the reference's own double arithmetic).  The Box-Muller
+// angle and the mean + v * stddev affine are evaluated with contraction off so
+// they round exactly as the reference host path does; the transcendental
+// functions are the accurate (non-native) device versions.
+#pragma once
+
+#include "rr_engines.hpp"
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace rr
+{
+
+// rocrand_common.h:24-37
+constexpr float  R_2POW16_INV     = 1.5258789e-05f;
+constexpr float  R_2POW16_INV_2PI = 9.58738e-05f;
+constexpr float  R_2POW32_INV     = 2.3283064e-10f;
+constexpr double R_2POW32_INV_DBL = 2.3283064365386963e-10;
+constexpr double R_2POW64_INV_DBL = 5.4210108624275221700372640043497e-20;
+constexpr float  R_2POW32_INV_2PI = 1.46291807e-09f;
+constexpr double R_2POW53_INV_DBL = 1.1102230246251565e-16;
+constexpr double R_PI_DBL         = 3.1415926535897932;
+constexpr float  R_2PI            = 6.2831855f;
+constexpr float  R_SQRT2          = 1.4142135f;
+
+// ---------------------------------------------------------------- scalar math
+// Float transcendentals evaluated in double and rounded once: correctly rounded
+// except in cases within 2^-29 relative of a rounding boundary, so each differs
+// from the host libm's float result by at most the host's own error (< 1 ulp).
+__device__ __forceinline__ float logf_acc(float x) { return (float)log((double)x); }
+__device__ __forceinline__ float expf_acc(float x) { return (float)exp((double)x); }
+__device__ __forceinline__ void  sincosf_acc(float x, float* s, float* c)
+{
+    double sd, cd;
+    sincos((double)x, &sd, &cd);
+    *s = (float)sd;
+    *c = (float)cd;
+}
+
+// rocrand_uniform.h:63-66: 2^-32 + v * 2^-32 (both steps exact up to the final add).
+__device__ __forceinline__ float uniform_f32(uint32_t v)
+{
+    return __fadd_rn(R_2POW32_INV, __fmul_rn((float)v, R_2POW32_INV));
+}
+// rocrand_uniform.h:70-73
+__device__ __forceinline__ float uniform_f32(uint64_t v)
+{
+    return __fadd_rn(R_2POW32_INV, __fmul_rn((float)(v >> 32), R_2POW32_INV));
+}
+// rocrand_uniform.h:100-107
+__device__ __forceinline__ double uniform_f64(uint32_t v1, uint32_t v2)
+{
+    const uint64_t v = (uint64_t)v1 | ((uint64_t)(v2 >> 11) << 32);
+    return __dadd_rn(R_2POW53_INV_DBL, __dmul_rn((double)v, R_2POW53_INV_DBL));
+}
+// rocrand_uniform.h:109-116
+__device__ __forceinline__ double uniform_f64(uint64_t v)
+{
+    return __dadd_rn(R_2POW53_INV_DBL, __dmul_rn((double)(v >> 11), R_2POW53_INV_DBL));
+}
+// rocrand_uniform.h:154-157
+__device__ __forceinline__ __half uniform_f16(uint32_t v16)
+{
+    return __float2half(__fadd_rn(R_2POW16_INV, __fmul_rn((float)(v16 & 0xffffu), R_2POW16_INV)));
+}
+
+template<int MRG>
+struct mrg_consts;
+template<>
+struct mrg_consts<32>
+{
+    static constexpr double norm      = MRG32_NORM_DOUBLE;
+    static constexpr double uint_norm = MRG32_UINT_NORM;
+};
+template<>
+struct mrg_consts<31>
+{
+    static constexpr double norm      = MRG31_NORM_DOUBLE;
+    static constexpr double uint_norm = MRG31_UINT_NORM;
+};
+
+// rocrand_uniform.h:161-219
+template<int MRG>
+__device__ __forceinline__ uint32_t mrg_uint(uint32_t v)
+{
+    return (uint32_t)__dmul_rn((double)(v - 1), mrg_consts<MRG>::uint_norm);
+}
+template<int MRG>
+__device__ __forceinline__ float mrg_f32(uint32_t v)
+{
+    return (float)__dmul_rn((double)v, mrg_consts<MRG>::norm);
+}
+template<int MRG>
+__device__ __forceinline__ double mrg_f64(uint32_t v)
+{
+    return __dmul_rn((double)v, mrg_consts<MRG>::norm);
+}
+
+// rocrand_normal.h:51-66 (host branch: separate sinf/cosf of the same angle).
+__device__ __forceinline__ void box_muller(uint32_t x, uint32_t y, float& r0, float& r1)
+{
+    const float u = uniform_f32(x);
+    const float v = __fadd_rn(R_2POW32_INV_2PI, __fmul_rn((float)y, R_2POW32_INV_2PI));
+    const float s = sqrtf(__fmul_rn(-2.0f, logf_acc(u)));
+    float       sn, cs;
+    sincosf_acc(v, &sn, &cs);
+    r0 = __fmul_rn(sn, s);
+    r1 = __fmul_rn(cs, s);
+}
+// rocrand_normal.h:76-96 (host branch: sin(w * pi), not sincospi).
+__device__ __forceinline__ void box_muller_double(uint32_t a, uint32_t b, uint32_t c, uint32_t d, double& r0,
+                                                  double& r1)
+{
+    const uint64_t v1 = (uint64_t)a ^ ((uint64_t)b << 21);
+    const double   u  = __dadd_rn(R_2POW53_INV_DBL, __dmul_rn((double)v1, R_2POW53_INV_DBL));
+    const uint64_t v2 = (uint64_t)c ^ ((uint64_t)d << 21);
+    const double   w  = __dadd_rn(R_2POW53_INV_DBL * 2.0, __dmul_rn((double)v2, R_2POW53_INV_DBL * 2.0));
+    const double   s  = sqrt(__dmul_rn(-2.0, log(u)));
+    const double   t  = __dmul_rn(w, R_PI_DBL);
+    double         sn, cs;
+    sincos(t, &sn, &cs);
+    r0 = __dmul_rn(sn, s);
+    r1 = __dmul_rn(cs, s);
+}
+// rocrand_normal.h:108-135, host (float math) branch.
+__device__ __forceinline__ void box_muller_half(uint32_t x16, uint32_t y16, float& h0, float& h1)
+{
+    const float u = __fadd_rn(R_2POW16_INV, __fmul_rn((float)(x16 & 0xffffu), R_2POW16_INV));
+    const float v = __fadd_rn(R_2POW16_INV_2PI, __fmul_rn((float)(y16 & 0xffffu), R_2POW16_INV_2PI));
+    const float s = sqrtf(__fmul_rn(-2.0f, logf_acc(u)));
+    float       sn, cs;
+    sincosf_acc(v, &sn, &cs);
+    h0 = __half2float(__float2half(__fmul_rn(sn, s)));
+    h1 = __half2float(__float2half(__fmul_rn(cs, s)));
+}
+// rocrand_normal.h:138-154
+template<int MRG>
+__device__ __forceinline__ void mrg_box_muller(uint32_t x, uint32_t y, float& r0, float& r1)
+{
+    const float u = mrg_f32<MRG>(x);
+    const float v = __fmul_rn(mrg_f32<MRG>(y), R_2PI);
+    const float s = sqrtf(__fmul_rn(-2.0f, logf_acc(u)));
+    float       sn, cs;
+    sincosf_acc(v, &sn, &cs);
+    r0 = __fmul_rn(sn, s);
+    r1 = __fmul_rn(cs, s);
+}
+// rocrand_normal.h:156-172: float-precision uniforms widened to double.
+template<int MRG>
+__device__ __forceinline__ void mrg_box_muller_double(uint32_t x, uint32_t y, double& r0, double& r1)
+{
+    const double u = (double)mrg_f32<MRG>(x);
+    const double v = __dmul_rn((double)mrg_f32<MRG>(y), 2.0);
+    const double s = sqrt(__dmul_rn(-2.0, log(u)));
+    double       sn, cs;
+    sincos(__dmul_rn(v, R_PI_DBL), &sn, &cs);
+    r0 = __dmul_rn(sn, s);
+    r1 = __dmul_rn(cs, s);
+}
+// rocrand_normal.h:199-221 roc_d_erfinv and :263-268 normal_distribution_double.
+__device__ __forceinline__ double roc_d_erfinv(double x)
+{
+    const double sgn = (x < 0.0) ? -1.0 : 1.0;
+    x                = __dmul_rn(__dsub_rn(1.0, x), __dadd_rn(1.0, x));
+    const double lnx = log(x);
+    if(isnan(lnx))
+        return 1.0;
+    if(isinf(lnx))
+        return 0.0;
+    const double tt1 = __dadd_rn(2.0 / (R_PI_DBL * 0.147), __dmul_rn(0.5, lnx));
+    const double tt2 = __dmul_rn(1.0 / 0.147, lnx);
+    return __dmul_rn(sgn, sqrt(__dadd_rn(-tt1, sqrt(__dsub_rn(__dmul_rn(tt1, tt1), tt2)))));
+}
+__device__ __forceinline__ double normal_double_1(double p)
+{
+    return __dmul_rn((double)R_SQRT2, roc_d_erfinv(__dsub_rn(__dmul_rn(2.0, p), 1.0)));
+}
+
+// ---------------------------------------------------------------- Poisson
+// Alias table (rocrand_discrete_types.h layout) sampled as rocrand_discrete.h:61-90.
+struct alias_table
+{
+    unsigned int        size;
+    unsigned int        offset;
+    const unsigned int* alias;
+    const double*       prob;
+};
+
+__device__ __forceinline__ unsigned int discrete_alias(double x, const alias_table& t)
+{
+    const double       nx  = __dmul_rn((double)t.size, x);
+    const double       fnx = floor(nx);
+    const double       y   = __dsub_rn(nx, fnx);
+    const unsigned int i   = (unsigned int)fnx;
+    return t.offset + (y < t.prob[i] ? i : t.alias[i]);
+}
+
+// ---------------------------------------------------------------- functors
+// Output kinds (one per rocrand_generate_* entry point).
+enum class out_kind : int
+{
+    u8,
+    u16,
+    u32,
+    u64,
+    uniform_f32,
+    uniform_f64,
+    uniform_f16,
+    normal_f32,
+    normal_f64,
+    normal_f16,
+    lognormal_f32,
+    lognormal_f64,
+    lognormal_f16,
+    poisson
+};
+
+// Runtime parameters of a transform (mean/stddev or Poisson table).
+struct dist_params
+{
+    float       mf, sf;
+    double      md, sd;
+    __half      mh, sh;
+    alias_table pt;
+    int         huge;
+    double      lambda, sqrt_lambda;
+};
+
+// Family of the engine feeding the transform: generic 32-bit words, generic
+// 64-bit words (ThreeFry x64), or MRG (raw values in [1, m1], remapped).
+enum class word_family : int
+{
+    w32,
+    w64,
+    mrg32,
+    mrg31
+};
+
+template<word_family F, out_kind K, bool NormalDoubleOneWord = false>
+struct dist;
+
+// Generic 32-bit engines (uniform.hpp:37-120, normal.hpp:45-108, log_normal.hpp:45-108).
+#define RR_DIST(F, K, IW, OW, T) \
+    template<bool B>              \
+    struct dist<F, K, B>          \
+    {                             \
+        static constexpr int in_w = IW, out_w = OW; \
+        using out_t               = T;              \
+        template<class W>                           \
+        __device__ __forceinline__ static void apply(const W* in, T* out, const dist_params& p)
+
+RR_DIST(word_family::w32, out_kind::u32, 1, 1, uint32_t) { (void)p; out[0] = (uint32_t)in[0]; } };
+RR_DIST(word_family::w32, out_kind::u8, 1, 4, uint8_t)
+{
+    (void)p;
+    const uint32_t v = (uint32_t)in[0];
+    out[0] = (uint8_t)v; out[1] = (uint8_t)(v >> 8); out[2] = (uint8_t)(v >> 16); out[3] = (uint8_t)(v >> 24);
+} };
+RR_DIST(word_family::w32, out_kind::u16, 1, 2, uint16_t)
+{
+    (void)p;
+    const uint32_t v = (uint32_t)in[0];
+    out[0] = (uint16_t)v; out[1] = (uint16_t)(v >> 16);
+} };
+RR_DIST(word_family::w32, out_kind::uniform_f32, 1, 1, float) { (void)p; out[0] = uniform_f32((uint32_t)in[0]); } };
+RR_DIST(word_family::w32, out_kind::uniform_f64, 2, 1, double)
+{
+    (void)p;
+    out[0] = uniform_f64((uint32_t)in[0], (uint32_t)in[1]);
+} };
+RR_DIST(word_family::w32, out_kind::uniform_f16, 1, 2, __half)
+{
+    (void)p;
+    const uint32_t v = (uint32_t)in[0];
+    out[0] = uniform_f16(v); out[1] = uniform_f16(v >> 16);
+} };
+RR_DIST(word_family::w32, out_kind::normal_f32, 2, 2, float)
+{
+    float r0, r1;
+    box_muller((uint32_t)in[0], (uint32_t)in[1], r0, r1);
+    out[0] = __fadd_rn(p.mf, __fmul_rn(r0, p.sf));
+    out[1] = __fadd_rn(p.mf, __fmul_rn(r1, p.sf));
+} };
+RR_DIST(word_family::w32, out_kind::lognormal_f32, 2, 2, float)
+{
+    float r0, r1;
+    box_muller((uint32_t)in[0], (uint32_t)in[1], r0, r1);
+    out[0] = expf_acc(__fadd_rn(p.mf, __fmul_rn(r0, p.sf)));
+    out[1] = expf_acc(__fadd_rn(p.mf, __fmul_rn(r1, p.sf)));
+} };
+RR_DIST(word_family::w32, out_kind::normal_f16, 1, 2, __half)
+{
+    float h0, h1;
+    const uint32_t v = (uint32_t)in[0];
+    box_muller_half(v, v >> 16, h0, h1);
+    const float m = __half2float(p.mh), s = __half2float(p.sh);
+    out[0] = __float2half(__fadd_rn(m, __fmul_rn(s, h0)));
+    out[1] = __float2half(__fadd_rn(m, __fmul_rn(s, h1)));
+} };
+RR_DIST(word_family::w32, out_kind::lognormal_f16, 1, 2, __half)
+{
+    float h0, h1;
+    const uint32_t v = (uint32_t)in[0];
+    box_muller_half(v, v >> 16, h0, h1);
+    const float m = __half2float(p.mh), s = __half2float(p.sh);
+    out[0] = __float2half(expf_acc(__fadd_rn(m, __fmul_rn(s, h0))));
+    out[1] = __float2half(expf_acc(__fadd_rn(m, __fmul_rn(s, h1))));
+} };
+RR_DIST(word_family::w32, out_kind::poisson, 1, 1, uint32_t)
+{
+    const uint32_t r = (uint32_t)in[0];
+    if(p.huge)
+    {
+        // poisson.hpp:80-108
+        const double nd = normal_double_1(__dadd_rn(R_2POW32_INV_DBL, __dmul_rn((double)r, R_2POW32_INV_DBL)));
+        out[0] = (uint32_t)round(__dadd_rn(__dmul_rn(p.sqrt_lambda, nd), p.lambda)) ;
+        return;
+    }
+    out[0] = discrete_alias(__dmul_rn((double)r, R_2POW32_INV_DBL), p.pt);
+} };
+
+// normal_f64 / lognormal_f64: four words per pair (default), or one word through
+// erfinv for ThreeFry2x32 (threefry.hpp:441-449, normal.hpp:65-79).
+template<>
+struct dist<word_family::w32, out_kind::normal_f64, false>
+{
+    static constexpr int in_w = 4, out_w = 2;
+    using out_t               = double;
+    template<class W>
+    __device__ __forceinline__ static void apply(const W* in, double* out, const dist_params& p)
+    {
+        double r0, r1;
+        box_muller_double((uint32_t)in[0], (uint32_t)in[1], (uint32_t)in[2], (uint32_t)in[3], r0, r1);
+        out[0] = __dadd_rn(p.md, __dmul_rn(r0, p.sd));
+        out[1] = __dadd_rn(p.md, __dmul_rn(r1, p.sd));
+    }
+};
+template<>
+struct dist<word_family::w32, out_kind::lognormal_f64, false>
+{
+    static constexpr int in_w = 4, out_w = 2;
+    using out_t               = double;
+    template<class W>
+    __device__ __forceinline__ static void apply(const W* in, double* out, const dist_params& p)
+    {
+        double r0, r1;
+        box_muller_double((uint32_t)in[0], (uint32_t)in[1], (uint32_t)in[2], (uint32_t)in[3], r0, r1);
+        out[0] = exp(__dadd_rn(p.md, __dmul_rn(r0, p.sd)));
+        out[1] = exp(__dadd_rn(p.md, __dmul_rn(r1, p.sd)));
+    }
+};
+template<>
+struct dist<word_family::w32, out_kind::normal_f64, true>
+{
+    static constexpr int in_w = 1, out_w = 1;
+    using out_t               = double;
+    template<class W>
+    __device__ __forceinline__ static void apply(const W* in, double* out, const dist_params& p)
+    {
+        const double v = normal_double_1(__dadd_rn(R_2POW32_INV_DBL, __dmul_rn((double)(uint32_t)in[0], R_2POW32_INV_DBL)));
+        out[0] = __dadd_rn(p.md, __dmul_rn(v, p.sd));
+    }
+};
+template<>
+struct dist<word_family::w32, out_kind::lognormal_f64, true>
+{
+    static constexpr int in_w = 1, out_w = 1;
+    using out_t               = double;
+    template<class W>
+    __device__ __forceinline__ static void apply(const W* in, double* out, const dist_params& p)
+    {
+        const double v = normal_double_1(__dadd_rn(R_2POW32_INV_DBL, __dmul_rn((double)(uint32_t)in[0], R_2POW32_INV_DBL)));
+        out[0] = exp(__dadd_rn(p.md, __dmul_rn(v, p.sd)));
+    }
+};
+
+// 64-bit-word engines (uniform.hpp:122-218, normal.hpp:113-198).
+RR_DIST(word_family::w64, out_kind::u64, 1, 1, uint64_t) { (void)p; out[0] = (uint64_t)in[0]; } };
+RR_DIST(word_family::w64, out_kind::u32, 1, 2, uint32_t)
+{
+    (void)p;
+    const uint64_t v = (uint64_t)in[0];
+    out[0] = (uint32_t)v; out[1] = (uint32_t)(v >> 32);
+} };
+RR_DIST(word_family::w64, out_kind::u16, 1, 4, uint16_t)
+{
+    (void)p;
+    const uint64_t v = (uint64_t)in[0];
+#pragma unroll
+    for(int i = 0; i < 4; i++) out[i] = (uint16_t)(v >> (16 * i));
+} };
+RR_DIST(word_family::w64, out_kind::u8, 1, 8, uint8_t)
+{
+    (void)p;
+    const uint64_t v = (uint64_t)in[0];
+#pragma unroll
+    for(int i = 0; i < 8; i++) out[i] = (uint8_t)(v >> (8 * i));
+} };
+RR_DIST(word_family::w64, out_kind::uniform_f32, 1, 1, float) { (void)p; out[0] = uniform_f32((uint64_t)in[0]); } };
+RR_DIST(word_family::w64, out_kind::uniform_f64, 1, 1, double) { (void)p; out[0] = uniform_f64((uint64_t)in[0]); } };
+RR_DIST(word_family::w64, out_kind::uniform_f16, 1, 4, __half)
+{
+    (void)p;
+    const uint64_t v = (uint64_t)in[0];
+#pragma unroll
+    for(int i = 0; i < 4; i++) out[i] = uniform_f16((uint32_t)(v >> (16 * i)));
+} };
+RR_DIST(word_family::w64, out_kind::normal_f32, 1, 2, float)
+{
+    float r0, r1;
+    const uint64_t v = (uint64_t)in[0];
+    box_muller((uint32_t)v, (uint32_t)(v >> 32), r0, r1);
+    out[0] = __fadd_rn(p.mf, __fmul_rn(r0, p.sf));
+    out[1] = __fadd_rn(p.mf, __fmul_rn(r1, p.sf));
+} };
+RR_DIST(word_family::w64, out_kind::lognormal_f32, 1, 2, float)
+{
+    float r0, r1;
+    const uint64_t v = (uint64_t)in[0];
+    box_muller((uint32_t)v, (uint32_t)(v >> 32), r0, r1);
+    out[0] = expf_acc(__fadd_rn(p.mf, __fmul_rn(r0, p.sf)));
+    out[1] = expf_acc(__fadd_rn(p.mf, __fmul_rn(r1, p.sf)));
+} };
+RR_DIST(word_family::w64, out_kind::normal_f64, 2, 2, double)
+{
+    double r0, r1;
+    const uint64_t a = (uint64_t)in[0], b = (uint64_t)in[1];
+    box_muller_double((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32), r0, r1);
+    out[0] = __dadd_rn(p.md, __dmul_rn(r0, p.sd));
+    out[1] = __dadd_rn(p.md, __dmul_rn(r1, p.sd));
+} };
+RR_DIST(word_family::w64, out_kind::lognormal_f64, 2, 2, double)
+{
+    double r0, r1;
+    const uint64_t a = (uint64_t)in[0], b = (uint64_t)in[1];
+    box_muller_double((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32), r0, r1);
+    out[0] = exp(__dadd_rn(p.md, __dmul_rn(r0, p.sd)));
+    out[1] = exp(__dadd_rn(p.md, __dmul_rn(r1, p.sd)));
+} };
+RR_DIST(word_family::w64, out_kind::normal_f16, 1, 2, __half)
+{
+    float h0, h1;
+    const uint64_t v = (uint64_t)in[0];
+    box_muller_half((uint32_t)v, (uint32_t)(v >> 32), h0, h1);
+    const float m = __half2float(p.mh), s = __half2float(p.sh);
+    out[0] = __float2half(__fadd_rn(m, __fmul_rn(s, h0)));
+    out[1] = __float2half(__fadd_rn(m, __fmul_rn(s, h1)));
+} };
+RR_DIST(word_family::w64, out_kind::lognormal_f16, 1, 2, __half)
+{
+    float h0, h1;
+    const uint64_t v = (uint64_t)in[0];
+    box_muller_half((uint32_t)v, (uint32_t)(v >> 32), h0, h1);
+    const float m = __half2float(p.mh), s = __half2float(p.sh);
+    out[0] = __float2half(expf_acc(__fadd_rn(m, __fmul_rn(s, h0))));
+    out[1] = __float2half(expf_acc(__fadd_rn(m, __fmul_rn(s, h1))));
+} };
+RR_DIST(word_family::w64, out_kind::poisson, 1, 1, uint32_t)
+{
+    const uint64_t r = (uint64_t)in[0];
+    if(p.huge)
+    {
+        const double nd = normal_double_1(uniform_f64(r));
+        out[0] = (uint32_t)round(__dadd_rn(__dmul_rn(p.sqrt_lambda, nd), p.lambda));
+        return;
+    }
+    out[0] = discrete_alias(__dmul_rn((double)r, R_2POW64_INV_DBL), p.pt);
+} };
+
+// MRG engines (uniform.hpp:226-311, normal.hpp:200-270, poisson.hpp:416-441).
+#define RR_MRG_DISTS(F, M)                                                                       \
+    RR_DIST(F, out_kind::u32, 1, 1, uint32_t) { (void)p; out[0] = mrg_uint<M>((uint32_t)in[0]); } }; \
+    RR_DIST(F, out_kind::u8, 1, 4, uint8_t)                                                      \
+    {                                                                                            \
+        (void)p;                                                                                 \
+        const uint32_t v = mrg_uint<M>((uint32_t)in[0]);                                         \
+        out[0] = (uint8_t)v; out[1] = (uint8_t)(v >> 8); out[2] = (uint8_t)(v >> 16); out[3] = (uint8_t)(v >> 24); \
+    } };                                                                                         \
+    RR_DIST(F, out_kind::u16, 1, 2, uint16_t)                                                    \
+    {                                                                                            \
+        (void)p;                                                                                 \
+        const uint32_t v = mrg_uint<M>((uint32_t)in[0]);                                         \
+        out[0] = (uint16_t)v; out[1] = (uint16_t)(v >> 16);                                      \
+    } };                                                                                         \
+    RR_DIST(F, out_kind::uniform_f32, 1, 1, float) { (void)p; out[0] = mrg_f32<M>((uint32_t)in[0]); } }; \
+    RR_DIST(F, out_kind::uniform_f64, 1, 1, double) { (void)p; out[0] = mrg_f64<M>((uint32_t)in[0]); } }; \
+    RR_DIST(F, out_kind::uniform_f16, 1, 2, __half)                                              \
+    {                                                                                            \
+        (void)p;                                                                                 \
+        const uint32_t v = mrg_uint<M>((uint32_t)in[0]);                                         \
+        out[0] = uniform_f16(v); out[1] = uniform_f16(v >> 16);                                  \
+    } };                                                                                         \
+    RR_DIST(F, out_kind::normal_f32, 2, 2, float)                                                \
+    {                                                                                            \
+        float r0, r1;                                                                            \
+        mrg_box_muller<M>((uint32_t)in[0], (uint32_t)in[1], r0, r1);                             \
+        out[0] = __fadd_rn(p.mf, __fmul_rn(r0, p.sf));                                           \
+        out[1] = __fadd_rn(p.mf, __fmul_rn(r1, p.sf));                                           \
+    } };                                                                                         \
+    RR_DIST(F, out_kind::lognormal_f32, 2, 2, float)                                             \
+    {                                                                                            \
+        float r0, r1;                                                                            \
+        mrg_box_muller<M>((uint32_t)in[0], (uint32_t)in[1], r0, r1);                             \
+        out[0] = expf_acc(__fadd_rn(p.mf, __fmul_rn(r0, p.sf)));                                     \
+        out[1] = expf_acc(__fadd_rn(p.mf, __fmul_rn(r1, p.sf)));                                     \
+    } };                                                                                         \
+    RR_DIST(F, out_kind::normal_f64, 2, 2, double)                                               \
+    {                                                                                            \
+        double r0, r1;                                                                           \
+        mrg_box_muller_double<M>((uint32_t)in[0], (uint32_t)in[1], r0, r1);                      \
+        out[0] = __dadd_rn(p.md, __dmul_rn(r0, p.sd));                                           \
+        out[1] = __dadd_rn(p.md, __dmul_rn(r1, p.sd));                                           \
+    } };                                                                                         \
+    RR_DIST(F, out_kind::lognormal_f64, 2, 2, double)                                            \
+    {                                                                                            \
+        double r0, r1;                                                                           \
+        mrg_box_muller_double<M>((uint32_t)in[0], (uint32_t)in[1], r0, r1);                      \
+        out[0] = exp(__dadd_rn(p.md, __dmul_rn(r0, p.sd)));                                      \
+        out[1] = exp(__dadd_rn(p.md, __dmul_rn(r1, p.sd)));                                      \
+    } };                                                                                         \
+    RR_DIST(F, out_kind::normal_f16, 1, 2, __half)                                               \
+    {                                                                                            \
+        float h0, h1;                                                                            \
+        const uint32_t v = mrg_uint<M>((uint32_t)in[0]);                                         \
+        box_muller_half(v, v >> 16, h0, h1);                                                     \
+        const float m = __half2float(p.mh), s = __half2float(p.sh);                              \
+        out[0] = __float2half(__fadd_rn(m, __fmul_rn(s, h0)));                                   \
+        out[1] = __float2half(__fadd_rn(m, __fmul_rn(s, h1)));                                   \
+    } };                                                                                         \
+    RR_DIST(F, out_kind::lognormal_f16, 1, 2, __half)                                            \
+    {                                                                                            \
+        float h0, h1;                                                                            \
+        const uint32_t v = mrg_uint<M>((uint32_t)in[0]);                                         \
+        box_muller_half(v, v >> 16, h0, h1);                                                     \
+        const float m = __half2float(p.mh), s = __half2float(p.sh);                              \
+        out[0] = __float2half(expf_acc(__fadd_rn(m, __fmul_rn(s, h0))));                             \
+        out[1] = __float2half(expf_acc(__fadd_rn(m, __fmul_rn(s, h1))));                             \
+    } };                                                                                         \
+    RR_DIST(F, out_kind::poisson, 1, 1, uint32_t)                                                \
+    {                                                                                            \
+        const uint32_t r = mrg_uint<M>((uint32_t)in[0]);                                         \
+        if(p.huge)                                                                               \
+        {                                                                                        \
+            const double nd = normal_double_1(__dadd_rn(R_2POW32_INV_DBL, __dmul_rn((double)r, R_2POW32_INV_DBL))); \
+            out[0] = (uint32_t)round(__dadd_rn(__dmul_rn(p.sqrt_lambda, nd), p.lambda));          \
+            return;                                                                              \
+        }                                                                                        \
+        out[0] = discrete_alias(__dmul_rn((double)r, R_2POW32_INV_DBL), p.pt);                   \
+    } };
+
+RR_MRG_DISTS(word_family::mrg32, 32)
+RR_MRG_DISTS(word_family::mrg31, 31)
+
+#undef RR_MRG_DISTS
+#undef RR_DIST
+
+} // namespace rr

@@ -1,0 +1,327 @@
+// rr_interleaved.hip -- MRG32k3a, MRG31k3p, XORWOW and LFSR113 bulk generation on gfx950.
+//
+// These engines are sequential, so the reference runs S = 131072 independent
+// engines (one per thread of its 512 x 256 grid) and interleaves them: output
+// vec v comes from engine (v + start) mod S at that engine's (v / S)-th step
+// (library/src/rng/mrg.hpp:65-157, xorwow.hpp:63-161, lfsr113.hpp:66-160).
+// The engine states stay resident in HBM between calls, structure-of-arrays
+// (word j of engine e at states[j*S + e]) so every state load/store and every
+// output store of a wavefront is one contiguous segment.
+//
+// Engine e is initialised by a jump-ahead kernel: the seeded state advanced by
+// e subsequences and then by offset/S + [e < offset mod S] steps, with matrix
+// powers (mod-m 3x3 for the MRGs, GF(2) for XORWOW / LFSR113) computed on the
+// host once per device (rr_tables.cpp).
+#include "rr_launch.hpp"
+
+namespace rr
+{
+namespace
+{
+
+constexpr uint32_t S = INTERLEAVED_ENGINES;
+
+template<inter_kind K>
+struct ieng;
+
+template<>
+struct ieng<inter_kind::mrg32k3a>
+{
+    static constexpr word_family F = word_family::mrg32;
+    mrg_state                    s;
+    __device__ void load(const uint32_t* st, uint32_t e)
+    {
+#pragma unroll
+        for(int j = 0; j < 3; j++)
+        {
+            s.g1[j] = st[j * S + e];
+            s.g2[j] = st[(3 + j) * S + e];
+        }
+    }
+    __device__ void store(uint32_t* st, uint32_t e) const
+    {
+#pragma unroll
+        for(int j = 0; j < 3; j++)
+        {
+            st[j * S + e]       = s.g1[j];
+            st[(3 + j) * S + e] = s.g2[j];
+        }
+    }
+    __host__ void set(const uint32_t* w)
+    {
+        for(int j = 0; j < 3; j++)
+        {
+            s.g1[j] = w[j];
+            s.g2[j] = w[3 + j];
+        }
+    }
+    __device__ uint32_t next() { return mrg32k3a_next(s); }
+    // Apply X^(sum of set bits) from a table of X^(2^i) (rocrand_mrg32k3a.h:204-263).
+    __device__ void jump(const uint32_t* tab, uint64_t v)
+    {
+        for(int i = 0; v > 0; v >>= 1, i++)
+            if(v & 1)
+            {
+                mat3_vec_mod(tab + (i * 2 + 0) * 9, s.g1, MRG32_M1);
+                mat3_vec_mod(tab + (i * 2 + 1) * 9, s.g2, MRG32_M2);
+            }
+    }
+    __device__ void init(const device_tables& t, uint32_t e, uint64_t off)
+    {
+        jump(t.mrg32_sub, e);
+        jump(t.mrg32_step, off);
+    }
+};
+
+template<>
+struct ieng<inter_kind::mrg31k3p> : ieng<inter_kind::mrg32k3a>
+{
+    static constexpr word_family F = word_family::mrg31;
+    __device__ uint32_t next() { return mrg31k3p_next(s); }
+    __device__ void jump(const uint32_t* tab, uint64_t v)
+    {
+        for(int i = 0; v > 0; v >>= 1, i++)
+            if(v & 1)
+            {
+                mat3_vec_mod(tab + (i * 2 + 0) * 9, s.g1, MRG31_M1);
+                mat3_vec_mod(tab + (i * 2 + 1) * 9, s.g2, MRG31_M2);
+            }
+    }
+    __device__ void init(const device_tables& t, uint32_t e, uint64_t off)
+    {
+        jump(t.mrg31_sub, e);
+        jump(t.mrg31_step, off);
+    }
+};
+
+// Base-4 digit expansion with matrix k = M^(c * 4^k) applied digit times
+// (rocrand_xorwow.h:186-200, rocrand_lfsr113.h jump()).
+template<int N>
+__device__ void gf2_jump(uint32_t* v, const uint32_t* mats, uint64_t n)
+{
+    for(int k = 0; n > 0; n >>= 2, k++)
+    {
+        const uint32_t d = (uint32_t)(n & 3u);
+        for(uint32_t i = 0; i < d; i++)
+            gf2_mat_vec<N>(mats + (size_t)k * 32 * N * N, v);
+    }
+}
+
+template<>
+struct ieng<inter_kind::xorwow>
+{
+    static constexpr word_family F = word_family::w32;
+    xorwow_state                 s;
+    __device__ void load(const uint32_t* st, uint32_t e)
+    {
+        s.d = st[e];
+#pragma unroll
+        for(int j = 0; j < 5; j++)
+            s.x[j] = st[(1 + j) * S + e];
+    }
+    __device__ void store(uint32_t* st, uint32_t e) const
+    {
+        st[e] = s.d;
+#pragma unroll
+        for(int j = 0; j < 5; j++)
+            st[(1 + j) * S + e] = s.x[j];
+    }
+    __host__ void set(const uint32_t* w)
+    {
+        s.d = w[0];
+        for(int j = 0; j < 5; j++)
+            s.x[j] = w[1 + j];
+    }
+    __device__ uint32_t next() { return xorwow_next(s); }
+    __device__ void init(const device_tables& t, uint32_t e, uint64_t off)
+    {
+        gf2_jump<5>(s.x, t.xorwow_seq, e);
+        gf2_jump<5>(s.x, t.xorwow_jump, off);
+        s.d += (uint32_t)off * 362437u; // rocrand_xorwow.h:134-145
+    }
+};
+
+template<>
+struct ieng<inter_kind::lfsr113>
+{
+    static constexpr word_family F = word_family::w32;
+    lfsr113_state                s;
+    __device__ void load(const uint32_t* st, uint32_t e)
+    {
+#pragma unroll
+        for(int j = 0; j < 4; j++)
+            s.z[j] = st[j * S + e];
+    }
+    __device__ void store(uint32_t* st, uint32_t e) const
+    {
+#pragma unroll
+        for(int j = 0; j < 4; j++)
+            st[j * S + e] = s.z[j];
+    }
+    __host__ void set(const uint32_t* w)
+    {
+        for(int j = 0; j < 4; j++)
+            s.z[j] = w[j];
+    }
+    __device__ uint32_t next() { return lfsr113_next(s); }
+    __device__ void init(const device_tables& t, uint32_t e, uint64_t off)
+    {
+        // lfsr113.hpp:45-62: the step count travels as an unsigned int.
+        gf2_jump<4>(s.z, t.lfsr_seq, (uint32_t)e);
+        gf2_jump<4>(s.z, t.lfsr_jump, (uint32_t)off);
+    }
+};
+
+template<inter_kind K>
+__global__ __launch_bounds__(256) void init_kernel(uint32_t* states, const device_tables t, ieng<K> seeded,
+                                                   uint32_t start, uint64_t base_off)
+{
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if(e >= S)
+        return;
+    ieng<K> g = seeded;
+    g.init(t, e, base_off + (e < start ? 1u : 0u));
+    g.store(states, e);
+}
+
+template<int BYTES>
+__device__ __forceinline__ void store_bytes(void* dst, const void* src)
+{
+    if constexpr(BYTES == 4)
+        *reinterpret_cast<uint32_t*>(dst) = *reinterpret_cast<const uint32_t*>(src);
+    else if constexpr(BYTES == 8)
+        *reinterpret_cast<uint2*>(dst) = *reinterpret_cast<const uint2*>(src);
+    else
+        *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(src);
+}
+
+template<inter_kind K, class D>
+__global__ __launch_bounds__(256) void generate_kernel(interleaved_call c)
+{
+    using out_t          = typename D::out_t;
+    constexpr int IN     = D::in_w;
+    constexpr int OUT    = D::out_w;
+    constexpr int BYTES  = OUT * (int)sizeof(out_t);
+    static_assert(BYTES == 4 || BYTES == 8 || BYTES == 16, "vec size");
+
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if(id >= S)
+        return;
+    const uint32_t eid = (id + c.start) % S;
+    ieng<K>        g;
+    g.load(c.states, eid);
+
+    out_t* const data     = static_cast<out_t*>(c.data);
+    out_t* const vec_data = data + c.mis;
+    alignas(16) out_t o[OUT];
+    uint32_t     in[IN];
+    size_t       index = id;
+    for(; index < c.vec_n; index += S)
+    {
+#pragma unroll
+        for(int i = 0; i < IN; i++)
+            in[i] = g.next();
+        D::apply(in, o, c.params);
+        store_bytes<BYTES>(vec_data + index * OUT, o);
+    }
+    // The engine that would produce vec index vec_n also writes the head and
+    // then the tail (mrg.hpp:114-153).
+    if(OUT > 1 && index == c.vec_n)
+    {
+        if(c.head > 0)
+        {
+#pragma unroll
+            for(int i = 0; i < IN; i++)
+                in[i] = g.next();
+            D::apply(in, o, c.params);
+            for(uint32_t j = 0; j < (uint32_t)OUT; j++)
+                if(j < c.head)
+                    data[j] = o[j];
+        }
+        if(c.tail > 0)
+        {
+#pragma unroll
+            for(int i = 0; i < IN; i++)
+                in[i] = g.next();
+            D::apply(in, o, c.params);
+            for(uint32_t j = 0; j < (uint32_t)OUT; j++)
+                if(j < c.tail)
+                    data[c.n - c.tail + j] = o[j];
+        }
+    }
+    g.store(c.states, eid);
+}
+
+template<inter_kind K, class D>
+hipError_t launch_gen(const interleaved_call& c, hipStream_t s)
+{
+    hipLaunchKernelGGL((generate_kernel<K, D>), dim3(S / 256), dim3(256), 0, s, c);
+    return hipGetLastError();
+}
+
+template<inter_kind K>
+hipError_t dispatch(out_kind o, const interleaved_call& c, hipStream_t s)
+{
+    constexpr word_family F = ieng<K>::F;
+    switch(o)
+    {
+        case out_kind::u8: return launch_gen<K, dist<F, out_kind::u8>>(c, s);
+        case out_kind::u16: return launch_gen<K, dist<F, out_kind::u16>>(c, s);
+        case out_kind::u32: return launch_gen<K, dist<F, out_kind::u32>>(c, s);
+        case out_kind::uniform_f32: return launch_gen<K, dist<F, out_kind::uniform_f32>>(c, s);
+        case out_kind::uniform_f64: return launch_gen<K, dist<F, out_kind::uniform_f64>>(c, s);
+        case out_kind::uniform_f16: return launch_gen<K, dist<F, out_kind::uniform_f16>>(c, s);
+        case out_kind::normal_f32: return launch_gen<K, dist<F, out_kind::normal_f32>>(c, s);
+        case out_kind::normal_f64: return launch_gen<K, dist<F, out_kind::normal_f64>>(c, s);
+        case out_kind::normal_f16: return launch_gen<K, dist<F, out_kind::normal_f16>>(c, s);
+        case out_kind::lognormal_f32: return launch_gen<K, dist<F, out_kind::lognormal_f32>>(c, s);
+        case out_kind::lognormal_f64: return launch_gen<K, dist<F, out_kind::lognormal_f64>>(c, s);
+        case out_kind::lognormal_f16: return launch_gen<K, dist<F, out_kind::lognormal_f16>>(c, s);
+        case out_kind::poisson: return launch_gen<K, dist<F, out_kind::poisson>>(c, s);
+        case out_kind::u64: return hipErrorInvalidValue;
+    }
+    return hipErrorInvalidValue;
+}
+
+template<inter_kind K>
+hipError_t launch_init_k(uint32_t* states, const uint32_t* seed_state, uint32_t start, uint64_t base_off,
+                         hipStream_t s)
+{
+    device_tables t;
+    hipError_t    err = get_device_tables(t);
+    if(err != hipSuccess)
+        return err;
+    ieng<K> seeded;
+    seeded.set(seed_state);
+    hipLaunchKernelGGL((init_kernel<K>), dim3(S / 256), dim3(256), 0, s, states, t, seeded, start, base_off);
+    return hipGetLastError();
+}
+
+} // namespace
+
+hipError_t launch_interleaved_init(inter_kind k, uint32_t* states, const uint32_t* seed_state, uint32_t start,
+                                   uint64_t base_offset, hipStream_t s)
+{
+    switch(k)
+    {
+        case inter_kind::mrg32k3a: return launch_init_k<inter_kind::mrg32k3a>(states, seed_state, start, base_offset, s);
+        case inter_kind::mrg31k3p: return launch_init_k<inter_kind::mrg31k3p>(states, seed_state, start, base_offset, s);
+        case inter_kind::xorwow: return launch_init_k<inter_kind::xorwow>(states, seed_state, start, base_offset, s);
+        case inter_kind::lfsr113: return launch_init_k<inter_kind::lfsr113>(states, seed_state, start, base_offset, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_interleaved(inter_kind k, out_kind o, const interleaved_call& c, hipStream_t s)
+{
+    switch(k)
+    {
+        case inter_kind::mrg32k3a: return dispatch<inter_kind::mrg32k3a>(o, c, s);
+        case inter_kind::mrg31k3p: return dispatch<inter_kind::mrg31k3p>(o, c, s);
+        case inter_kind::xorwow: return dispatch<inter_kind::xorwow>(o, c, s);
+        case inter_kind::lfsr113: return dispatch<inter_kind::lfsr113>(o, c, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+} // namespace rr

@@ -1,0 +1,93 @@
+// rr_launch.hpp -- host-side launch interface between the generator objects and
+// the gfx950 kernels (one translation unit per engine family).
+#pragma once
+
+#include "rr_dist.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <stddef.h>
+#include <stdint.h>
+
+namespace rr
+{
+
+// Grid of the reference's DEFAULT ordering for the counter-based engines
+// (config_defaults.hpp:54-108): 1024 blocks x 256 threads.  It fixes which vec
+// indices take the head-shifted engine position (see counter_call).
+constexpr uint32_t CTR_REF_STRIDE = 1024u * 256u;
+// Engines of the interleaved generators (config_defaults.hpp:33-73): 512 x 256.
+constexpr uint32_t INTERLEAVED_ENGINES = 512u * 256u;
+
+// One counter-based generate call, fully resolved on the host.
+struct counter_call
+{
+    ctr_key  key;
+    ctr_pos  pos;        // word position of the call's first word (engine state)
+    ctr_pos  vec_base;   // position of vec 0's first word: pos + H
+    ctr_pos  vec_base0;  // position used by vec indices that are multiples of CTR_REF_STRIDE: pos + C0
+    int      quirk;      // vec_base0 != vec_base
+    void*    data;       // output base pointer
+    size_t   n;          // number of outputs
+    size_t   mis;        // misalignment (outputs before the first full vec)
+    uint32_t head, tail; // head/tail output counts
+    size_t   vec_n;      // full vecs
+    dist_params params;
+};
+
+hipError_t launch_counter(ctr_kind k, out_kind o, bool normal_one_word, const counter_call& c, hipStream_t s);
+
+// One interleaved generate call.
+struct interleaved_call
+{
+    uint32_t*   states;   // structure-of-arrays engine states, word j of engine e at [j*S + e]
+    uint32_t    start;    // start engine
+    void*       data;
+    size_t      n, mis;
+    uint32_t    head, tail;
+    size_t      vec_n;
+    dist_params params;
+};
+
+enum class inter_kind : int
+{
+    mrg32k3a = 0,
+    mrg31k3p = 1,
+    xorwow   = 2,
+    lfsr113  = 3
+};
+
+// Number of 32-bit state words per engine in the SoA layout.
+constexpr int inter_state_words(inter_kind k)
+{
+    return k == inter_kind::lfsr113 ? 4 : 6;
+}
+
+hipError_t launch_interleaved_init(inter_kind k,
+                                   uint32_t*  states,
+                                   const uint32_t* seed_state, // one engine's seeded state
+                                   uint32_t   start,
+                                   uint64_t   base_offset,
+                                   hipStream_t s);
+hipError_t launch_interleaved(inter_kind k, out_kind o, const interleaved_call& c, hipStream_t s);
+
+// Host-computed jump tables, uploaded once per device (rr_tables.cpp).
+struct device_tables
+{
+    const uint32_t* mrg32_step; // [64][2][9]  A^(2^i)
+    const uint32_t* mrg32_sub;  // [64][2][9]  A^(2^(76+i))
+    const uint32_t* mrg31_step;
+    const uint32_t* mrg31_sub;  // A^(2^(72+i))
+    const uint32_t* xorwow_jump; // [32][800]  M^(4^k)
+    const uint32_t* xorwow_seq;  // [32][800]  M^(2^67 * 4^k)
+    const uint32_t* lfsr_jump;   // [32][512]
+    const uint32_t* lfsr_seq;    // [32][512]  M^(2^55 * 4^k)
+};
+// Returns the tables of the current device (built and uploaded on first use).
+hipError_t get_device_tables(device_tables& t);
+
+// Host copies of the same tables (used by host-side seeding helpers and tests).
+const uint32_t* host_mrg_table(int which_engine /*32 or 31*/, int sub /*0 step,1 subsequence*/);
+const uint32_t* host_gf2_table(int xorwow /*1 xorwow, 0 lfsr*/, int seq);
+
+} // namespace rr

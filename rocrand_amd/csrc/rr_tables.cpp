@@ -1,0 +1,233 @@
+// rr_tables.cpp -- jump-ahead tables of the stateful engines, computed on the host
+// from the one-step recurrences and uploaded once per device.
+//
+// The reference ships these as generated headers (rocrand_mrg32k3a_precomputed.h,
+// rocrand_mrg31k3p_precomputed.h, rocrand_xorwow_precomputed.h,
+// rocrand_lfsr113_precomputed.h, written by tools/*_precomputed_generator.cpp).
+// They are pure matrix powers, so they are derived here instead of carried:
+//   MRG32k3a: A^(2^i) and A^(2^(76+i))       (subsequence = 2^76 steps)
+//   MRG31k3p: A^(2^i) and A^(2^(72+i))       (subsequence = 2^72 steps)
+//   XORWOW  : M^(4^k) and M^(2^67 * 4^k)     (GF(2), 160x160)
+//   LFSR113 : M^(4^k) and M^(2^55 * 4^k)     (GF(2), 128x128)
+#include "rr_launch.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string.h>
+#include <vector>
+
+namespace rr
+{
+namespace
+{
+
+// ---- mod-m 3x3 matrices (row-major, entries < m)
+void mat3_mul(const uint64_t* A, const uint64_t* B, uint64_t* C, uint64_t m)
+{
+    uint64_t T[9];
+    for(int r = 0; r < 3; r++)
+        for(int c = 0; c < 3; c++)
+        {
+            uint64_t acc = 0;
+            for(int k = 0; k < 3; k++)
+                acc += (A[3 * r + k] * B[3 * k + c]) % m;
+            T[3 * r + c] = acc % m;
+        }
+    memcpy(C, T, sizeof(T));
+}
+
+// table[i][comp][9] = X_comp^(2^(first + i)), i = 0..63
+void mrg_table(const uint64_t A1[9], const uint64_t A2[9], uint64_t m1, uint64_t m2, int first, uint32_t* out)
+{
+    const uint64_t* A[2] = {A1, A2};
+    const uint64_t  m[2] = {m1, m2};
+    for(int c = 0; c < 2; c++)
+    {
+        uint64_t X[9];
+        memcpy(X, A[c], sizeof(X));
+        for(int i = 0; i < first; i++)
+            mat3_mul(X, X, X, m[c]);
+        for(int i = 0; i < 64; i++)
+        {
+            for(int j = 0; j < 9; j++)
+                out[(i * 2 + c) * 9 + j] = (uint32_t)X[j];
+            mat3_mul(X, X, X, m[c]);
+        }
+    }
+}
+
+// ---- GF(2) matrices: row (word i, bit j) holds the N-word image of that input bit.
+template<int N>
+void gf2_apply(const uint32_t* m, uint32_t* v)
+{
+    uint32_t r[N] = {};
+    for(int i = 0; i < N; i++)
+        for(int j = 0; j < 32; j++)
+            if((v[i] >> j) & 1u)
+                for(int k = 0; k < N; k++)
+                    r[k] ^= m[(i * 32 + j) * N + k];
+    memcpy(v, r, sizeof(r));
+}
+template<int N>
+void gf2_mul(const uint32_t* A, const uint32_t* B, uint32_t* C) // apply A then B
+{
+    std::vector<uint32_t> T(32 * N * N);
+    for(int row = 0; row < 32 * N; row++)
+    {
+        uint32_t v[N];
+        memcpy(v, A + row * N, sizeof(v));
+        gf2_apply<N>(B, v);
+        memcpy(&T[row * N], v, sizeof(v));
+    }
+    memcpy(C, T.data(), T.size() * sizeof(uint32_t));
+}
+template<int N>
+void gf2_tables(void (*step)(uint32_t*), int seq_log2, uint32_t* jump, uint32_t* seq)
+{
+    const int             SZ = 32 * N * N;
+    std::vector<uint32_t> one(SZ), a(SZ);
+    for(int i = 0; i < N; i++)
+        for(int j = 0; j < 32; j++)
+        {
+            uint32_t v[N] = {};
+            v[i]          = 1u << j;
+            step(v);
+            memcpy(&one[(i * 32 + j) * N], v, sizeof(v));
+        }
+    a = one;
+    for(int k = 0; k < 32; k++)
+    {
+        memcpy(jump + k * SZ, a.data(), SZ * 4);
+        gf2_mul<N>(a.data(), a.data(), a.data());
+        gf2_mul<N>(a.data(), a.data(), a.data());
+    }
+    a = one;
+    for(int s = 0; s < seq_log2; s++)
+        gf2_mul<N>(a.data(), a.data(), a.data());
+    for(int k = 0; k < 32; k++)
+    {
+        memcpy(seq + k * SZ, a.data(), SZ * 4);
+        gf2_mul<N>(a.data(), a.data(), a.data());
+        gf2_mul<N>(a.data(), a.data(), a.data());
+    }
+}
+
+void xorwow_x_step(uint32_t* x)
+{
+    xorwow_state s{0, {x[0], x[1], x[2], x[3], x[4]}};
+    xorwow_next(s);
+    memcpy(x, s.x, 5 * 4);
+}
+void lfsr_step(uint32_t* z) { lfsr113_step(z); }
+
+struct host_tables
+{
+    std::vector<uint32_t> mrg32_step, mrg32_sub, mrg31_step, mrg31_sub;
+    std::vector<uint32_t> xorwow_jump, xorwow_seq, lfsr_jump, lfsr_seq;
+    host_tables()
+    {
+        mrg32_step.resize(64 * 18);
+        mrg32_sub.resize(64 * 18);
+        mrg31_step.resize(64 * 18);
+        mrg31_sub.resize(64 * 18);
+        {
+            // MRG32k3a one-step matrices, state (oldest .. newest), rocrand_mrg32k3a.h:160-193
+            const uint64_t A1[9] = {0, 1, 0, 0, 0, 1, MRG32_M1 - MRG32_A13N, MRG32_A12, 0};
+            const uint64_t A2[9] = {0, 1, 0, 0, 0, 1, MRG32_M2 - MRG32_A23N, 0, MRG32_A21};
+            mrg_table(A1, A2, MRG32_M1, MRG32_M2, 0, mrg32_step.data());
+            mrg_table(A1, A2, MRG32_M1, MRG32_M2, 76, mrg32_sub.data());
+        }
+        {
+            // MRG31k3p one-step matrices, state (newest .. oldest), rocrand_mrg31k3p.h next()
+            const uint64_t A1[9] = {0, 1ull << 22, (1ull << 7) + 1, 1, 0, 0, 0, 1, 0};
+            const uint64_t A2[9] = {1ull << 15, 0, (1ull << 15) + 1, 1, 0, 0, 0, 1, 0};
+            mrg_table(A1, A2, MRG31_M1, MRG31_M2, 0, mrg31_step.data());
+            mrg_table(A1, A2, MRG31_M1, MRG31_M2, 72, mrg31_sub.data());
+        }
+        xorwow_jump.resize(32 * 800);
+        xorwow_seq.resize(32 * 800);
+        gf2_tables<5>(xorwow_x_step, 67, xorwow_jump.data(), xorwow_seq.data());
+        lfsr_jump.resize(32 * 512);
+        lfsr_seq.resize(32 * 512);
+        gf2_tables<4>(lfsr_step, 55, lfsr_jump.data(), lfsr_seq.data());
+    }
+};
+
+const host_tables& host()
+{
+    static host_tables t;
+    return t;
+}
+
+struct device_cache
+{
+    std::mutex    mu;
+    bool          ready[64] = {};
+    device_tables t[64];
+};
+device_cache& cache()
+{
+    static device_cache c;
+    return c;
+}
+
+hipError_t upload(const std::vector<uint32_t>& v, const uint32_t** out)
+{
+    void*      p   = nullptr;
+    hipError_t err = hipMalloc(&p, v.size() * 4);
+    if(err != hipSuccess)
+        return err;
+    err = hipMemcpy(p, v.data(), v.size() * 4, hipMemcpyHostToDevice);
+    *out = static_cast<const uint32_t*>(p);
+    return err;
+}
+
+} // namespace
+
+hipError_t get_device_tables(device_tables& t)
+{
+    int        dev = 0;
+    hipError_t err = hipGetDevice(&dev);
+    if(err != hipSuccess)
+        return err;
+    if(dev < 0 || dev >= 64)
+        return hipErrorInvalidDevice;
+    device_cache&               c = cache();
+    std::lock_guard<std::mutex> lock(c.mu);
+    if(!c.ready[dev])
+    {
+        const host_tables& h = host();
+        device_tables      d{};
+        if((err = upload(h.mrg32_step, &d.mrg32_step)) != hipSuccess) return err;
+        if((err = upload(h.mrg32_sub, &d.mrg32_sub)) != hipSuccess) return err;
+        if((err = upload(h.mrg31_step, &d.mrg31_step)) != hipSuccess) return err;
+        if((err = upload(h.mrg31_sub, &d.mrg31_sub)) != hipSuccess) return err;
+        if((err = upload(h.xorwow_jump, &d.xorwow_jump)) != hipSuccess) return err;
+        if((err = upload(h.xorwow_seq, &d.xorwow_seq)) != hipSuccess) return err;
+        if((err = upload(h.lfsr_jump, &d.lfsr_jump)) != hipSuccess) return err;
+        if((err = upload(h.lfsr_seq, &d.lfsr_seq)) != hipSuccess) return err;
+        c.t[dev]     = d;
+        c.ready[dev] = true;
+    }
+    t = c.t[dev];
+    return hipSuccess;
+}
+
+const uint32_t* host_mrg_table(int which_engine, int sub)
+{
+    const host_tables& h = host();
+    if(which_engine == 32)
+        return sub ? h.mrg32_sub.data() : h.mrg32_step.data();
+    return sub ? h.mrg31_sub.data() : h.mrg31_step.data();
+}
+
+const uint32_t* host_gf2_table(int xorwow, int seq)
+{
+    const host_tables& h = host();
+    if(xorwow)
+        return seq ? h.xorwow_seq.data() : h.xorwow_jump.data();
+    return seq ? h.lfsr_seq.data() : h.lfsr_jump.data();
+}
+
+} // namespace rr
