@@ -8,8 +8,10 @@
 // position + C0 (the words the head actually consumed).  The reference's grid is
 // not this kernel's grid: any schedule that computes that map is equivalent.
 //
-// Schedule: grid-stride over vecs, each lane producing one vec per iteration so
-// a wavefront writes 64 consecutive vecs (1-2 KiB) with 16-byte stores.
+// Schedule: one vec per lane over a grid that covers the whole call, so the
+// resident wavefronts always write one contiguous front of the output (16-byte
+// stores, 1 KiB per wavefront).  A grid-stride loop over a small grid spreads the
+// concurrent stores over the whole buffer and was measured 1.5x slower on MI355X.
 #include "rr_launch.hpp"
 
 namespace rr
@@ -97,6 +99,61 @@ __device__ __forceinline__ void store_vec(void* dst, const uint32_t* u)
     }
 }
 
+// Head and tail of a call (at most FULL-1 outputs each), written by one thread.
+template<ctr_kind K, class D>
+__device__ void head_tail(const counter_call& c)
+{
+    using T                    = ctr_traits<K>;
+    using W                    = typename T::word;
+    using out_t                = typename D::out_t;
+    constexpr int DIM          = T::DIM;
+    constexpr int IN           = D::in_w;
+    constexpr int OUT          = D::out_w;
+    constexpr int OPT          = DIM / IN;
+    out_t* const  data         = static_cast<out_t*>(c.data);
+    {
+        out_t tmp[OUT];
+        if(c.head > 0)
+        {
+            // Head: consecutive words from the engine position (thread 0 of the reference grid).
+            seq_reader<K> r(c.pos, 0, c.key);
+            for(int s = 0; s < OPT; s++)
+            {
+                if((uint32_t)(s * OUT) >= c.head)
+                    break;
+                W in[IN];
+#pragma unroll
+                for(int i = 0; i < IN; i++)
+                    in[i] = r.next(c.key);
+                D::apply(in, tmp, c.params);
+                for(int o = 0; o < OUT; o++)
+                    if((uint32_t)(s * OUT + o) < c.head)
+                        data[s * OUT + o] = tmp[o];
+            }
+        }
+        if(c.tail > 0)
+        {
+            // Tail: the thread whose next vec index is vec_n continues from that vec's position.
+            const ctr_pos& b = (c.quirk && (c.vec_n % CTR_REF_STRIDE) == 0) ? c.vec_base0 : c.vec_base;
+            seq_reader<K>  r(b, c.vec_n, c.key);
+            for(int s = 0; s < OPT; s++)
+            {
+                if((uint32_t)(s * OUT) >= c.tail)
+                    break;
+                W in[IN];
+#pragma unroll
+                for(int i = 0; i < IN; i++)
+                    in[i] = r.next(c.key);
+                D::apply(in, tmp, c.params);
+                for(int o = 0; o < OUT; o++)
+                    if((uint32_t)(s * OUT + o) < c.tail)
+                        data[c.n - c.tail + s * OUT + o] = tmp[o];
+            }
+        }
+    }
+
+}
+
 template<ctr_kind K, class D>
 __global__ __launch_bounds__(256) void counter_kernel(counter_call c)
 {
@@ -150,55 +207,114 @@ __global__ __launch_bounds__(256) void counter_kernel(counter_call c)
     }
 
     if(blockIdx.x == 0 && threadIdx.x == 0)
+        head_tail<K, D>(c);
+}
+
+
+// Fast path for the 32-bit-word engines when every vec shares one engine position
+// modulo the block (no head-shift quirk) and the counter's words above bit 63 do
+// not change within the call: the base lives in scalar registers and each lane
+// runs exactly one engine block per vec.  When the stream starts mid-block
+// (SUB), a wavefront covers 63 vecs with 64 consecutive blocks and each lane
+// takes the words it is missing from its neighbour's block (one cross-lane
+// shuffle per word) instead of running a second block.
+template<ctr_kind K, class D, bool SUB>
+__global__ __launch_bounds__(256) void counter_fast_kernel(counter_call c)
+{
+    using T                = ctr_traits<K>;
+    using W                = typename T::word;
+    using out_t            = typename D::out_t;
+    constexpr int DIM      = T::DIM;
+    constexpr int IN       = D::in_w;
+    constexpr int OUT      = D::out_w;
+    constexpr int OPT      = DIM / IN;
+    constexpr int FULL     = OPT * OUT;
+    constexpr int BYTES    = FULL * (int)sizeof(out_t);
+    constexpr uint32_t TILE = SUB ? 63u : 64u;
+    static_assert(sizeof(W) == 4, "32-bit-word engines only");
+
+    const uint64_t clo  = c.vec_base.c[0] | (c.vec_base.c[1] << 32);
+    const W        c2   = (W)c.vec_base.c[2];
+    const W        c3   = (W)c.vec_base.c[3];
+    const uint32_t sub  = c.vec_base.sub;
+    const uint32_t lane = threadIdx.x & 63u;
+    const size_t   wave   = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    out_t* const   vec_data = static_cast<out_t*>(c.data) + c.mis;
+
+    if(wave * TILE < c.vec_n)
     {
-        out_t tmp[OUT];
-        if(c.head > 0)
+        const size_t   v  = wave * TILE + lane;
+        const uint64_t cc = clo + v;
+        W              ctr[DIM];
+        ctr[0] = (W)cc;
+        ctr[1] = (W)(cc >> 32);
+        if constexpr(DIM == 4)
         {
-            // Head: consecutive words from the engine position (thread 0 of the reference grid).
-            seq_reader<K> r(c.pos, 0, c.key);
-            for(int s = 0; s < OPT; s++)
-            {
-                if((uint32_t)(s * OUT) >= c.head)
-                    break;
-                W in[IN];
-#pragma unroll
-                for(int i = 0; i < IN; i++)
-                    in[i] = r.next(c.key);
-                D::apply(in, tmp, c.params);
-                for(int o = 0; o < OUT; o++)
-                    if((uint32_t)(s * OUT + o) < c.head)
-                        data[s * OUT + o] = tmp[o];
-            }
+            ctr[2] = c2;
+            ctr[3] = c3;
         }
-        if(c.tail > 0)
+        W w[DIM];
+        T::block(ctr, c.key, w);
+        W words[DIM];
+        if constexpr(SUB)
         {
-            // Tail: the thread whose next vec index is vec_n continues from that vec's position.
-            const ctr_pos& b = (c.quirk && (c.vec_n % CTR_REF_STRIDE) == 0) ? c.vec_base0 : c.vec_base;
-            seq_reader<K>  r(b, c.vec_n, c.key);
-            for(int s = 0; s < OPT; s++)
-            {
-                if((uint32_t)(s * OUT) >= c.tail)
-                    break;
-                W in[IN];
+            W nx[DIM];
 #pragma unroll
-                for(int i = 0; i < IN; i++)
-                    in[i] = r.next(c.key);
-                D::apply(in, tmp, c.params);
-                for(int o = 0; o < OUT; o++)
-                    if((uint32_t)(s * OUT + o) < c.tail)
-                        data[c.n - c.tail + s * OUT + o] = tmp[o];
-            }
+            for(int i = 0; i < DIM; i++)
+                nx[i] = __shfl_down(w[i], 1, 64);
+            interleave<W, DIM>(w, nx, sub, words);
+        }
+        else
+        {
+#pragma unroll
+            for(int i = 0; i < DIM; i++)
+                words[i] = w[i];
+        }
+        if(v < c.vec_n && (!SUB || lane != 63u))
+        {
+            out_t o[FULL];
+#pragma unroll
+            for(int s = 0; s < OPT; s++)
+                D::apply(words + s * IN, o + s * OUT, c.params);
+            uint32_t u[BYTES / 4];
+            __builtin_memcpy(u, o, BYTES);
+            store_vec<BYTES>(vec_data + v * FULL, u);
         }
     }
+
+    if(blockIdx.x == 0 && threadIdx.x == 0)
+        head_tail<K, D>(c);
 }
 
 template<ctr_kind K, class D>
 hipError_t launch_one(const counter_call& c, hipStream_t s)
 {
-    const size_t max_blocks = 8192;
-    size_t       blocks     = (c.vec_n + 255) / 256;
-    if(blocks > max_blocks)
-        blocks = max_blocks;
+    using W = typename ctr_traits<K>::word;
+    if constexpr(sizeof(W) == 4)
+    {
+        // Fast path unless the head shifted some vecs (quirk) or the counter's low
+        // 64 bits would carry within the call (4-word engines only).
+        const uint64_t clo     = c.vec_base.c[0] | (c.vec_base.c[1] << 32);
+        const bool     carries = ctr_traits<K>::DIM == 4 && clo + c.vec_n + 1 < clo;
+        if(!c.quirk && !carries)
+        {
+            const uint32_t tile   = c.vec_base.sub ? 63u : 64u;
+            const size_t   tiles  = (c.vec_n + tile - 1) / tile; // one wavefront per tile
+            size_t         blocks = (tiles + 3) / 4;
+            if(blocks == 0)
+                blocks = 1;
+            if(blocks > 0x7fffffffu)
+                return hipErrorInvalidConfiguration;
+            if(c.vec_base.sub)
+                hipLaunchKernelGGL((counter_fast_kernel<K, D, true>), dim3((unsigned)blocks), dim3(256), 0, s, c);
+            else
+                hipLaunchKernelGGL((counter_fast_kernel<K, D, false>), dim3((unsigned)blocks), dim3(256), 0, s, c);
+            return hipGetLastError();
+        }
+    }
+    size_t blocks = (c.vec_n + 255) / 256;
+    if(blocks > 0x7fffffffu)
+        blocks = 0x7fffffffu; // the kernel strides over what one grid cannot cover
     if(blocks == 0)
         blocks = 1;
     hipLaunchKernelGGL((counter_kernel<K, D>), dim3((unsigned)blocks), dim3(256), 0, s, c);

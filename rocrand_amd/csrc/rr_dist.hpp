@@ -49,31 +49,32 @@ __device__ __forceinline__ void  sincosf_acc(float x, float* s, float* c)
     *c = (float)cd;
 }
 
-// rocrand_uniform.h:63-66: 2^-32 + v * 2^-32 (both steps exact up to the final add).
+// rocrand_uniform.h:63-66: 2^-32 + v * 2^-32.  The product by a power of two is
+// exact, so one fused multiply-add rounds exactly like the separate operations.
 __device__ __forceinline__ float uniform_f32(uint32_t v)
 {
-    return __fadd_rn(R_2POW32_INV, __fmul_rn((float)v, R_2POW32_INV));
+    return __builtin_fmaf((float)v, R_2POW32_INV, R_2POW32_INV);
 }
 // rocrand_uniform.h:70-73
 __device__ __forceinline__ float uniform_f32(uint64_t v)
 {
-    return __fadd_rn(R_2POW32_INV, __fmul_rn((float)(v >> 32), R_2POW32_INV));
+    return __builtin_fmaf((float)(v >> 32), R_2POW32_INV, R_2POW32_INV);
 }
-// rocrand_uniform.h:100-107
+// rocrand_uniform.h:100-107 (53-bit integer: exact in double, exact product)
 __device__ __forceinline__ double uniform_f64(uint32_t v1, uint32_t v2)
 {
     const uint64_t v = (uint64_t)v1 | ((uint64_t)(v2 >> 11) << 32);
-    return __dadd_rn(R_2POW53_INV_DBL, __dmul_rn((double)v, R_2POW53_INV_DBL));
+    return __builtin_fma((double)v, R_2POW53_INV_DBL, R_2POW53_INV_DBL);
 }
 // rocrand_uniform.h:109-116
 __device__ __forceinline__ double uniform_f64(uint64_t v)
 {
-    return __dadd_rn(R_2POW53_INV_DBL, __dmul_rn((double)(v >> 11), R_2POW53_INV_DBL));
+    return __builtin_fma((double)(v >> 11), R_2POW53_INV_DBL, R_2POW53_INV_DBL);
 }
 // rocrand_uniform.h:154-157
 __device__ __forceinline__ __half uniform_f16(uint32_t v16)
 {
-    return __float2half(__fadd_rn(R_2POW16_INV, __fmul_rn((float)(v16 & 0xffffu), R_2POW16_INV)));
+    return __float2half(__builtin_fmaf((float)(v16 & 0xffffu), R_2POW16_INV, R_2POW16_INV));
 }
 
 template<int MRG>

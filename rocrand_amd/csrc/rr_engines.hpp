@@ -31,6 +31,16 @@ struct q4
     uint64_t v[4];
 };
 
+// a ^ b ^ c in one instruction on gfx950 (v_bitop3_b32, truth table 0x96).
+__host__ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+#if defined(__HIP_DEVICE_COMPILE__) && __has_builtin(__builtin_amdgcn_bitop3_b32)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+
 __host__ __device__ __forceinline__ void philox_block(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                                       uint32_t k0, uint32_t k1, uint32_t out[4])
 {
@@ -41,8 +51,8 @@ __host__ __device__ __forceinline__ void philox_block(uint32_t c0, uint32_t c1, 
         // One 32x32->64 multiply per lane word: v_mad_u64_u32 on gfx950.
         const uint64_t p0 = (uint64_t)PHILOX_M0 * x;
         const uint64_t p1 = (uint64_t)PHILOX_M1 * z;
-        const uint32_t nx = (uint32_t)(p1 >> 32) ^ y ^ k0;
-        const uint32_t nz = (uint32_t)(p0 >> 32) ^ w ^ k1;
+        const uint32_t nx = xor3((uint32_t)(p1 >> 32), y, k0);
+        const uint32_t nz = xor3((uint32_t)(p0 >> 32), w, k1);
         y                 = (uint32_t)p1;
         w                 = (uint32_t)p0;
         x                 = nx;
