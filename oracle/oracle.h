@@ -98,8 +98,13 @@ unsigned short orc_float2half(float f);
 int orc_sobol_load(const char* joe_kuo_path);
 const unsigned int*       orc_sobol32_vectors(void);
 const unsigned long long* orc_sobol64_vectors(void);
+const unsigned int*       orc_scrambled_sobol32_vectors(void);
+const unsigned long long* orc_scrambled_sobol64_vectors(void);
 const unsigned int*       orc_scramble32_constants(void);
 const unsigned long long* orc_scramble64_constants(void);
+
+/* MTGP32 parameter sets (512 x {pos, sh1, sh2, tbl[16], tmp_tbl[16], mask}). */
+int orc_mtgp32_load(const char* params_path);
 
 #ifdef __cplusplus
 }

@@ -15,6 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libref.so")
+DATA = os.path.join(os.path.dirname(HERE), "rocrand_amd", "data")
 
 # rocrand_rng_type values (library/include/rocrand/rocrand.h:77-101)
 XORWOW, MRG32K3A, MTGP32, PHILOX4_32_10, MRG31K3P, LFSR113, MT19937 = 401, 402, 403, 404, 405, 406, 407
@@ -75,11 +76,30 @@ def lib():
                          ("orc_threefry2x64_20_block", ctypes.c_ulonglong)]:
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_sobol_load.argtypes = [ctypes.c_char_p]
-        L.orc_sobol32_vectors.restype = ctypes.POINTER(ctypes.c_uint)
-        L.orc_sobol64_vectors.restype = ctypes.POINTER(ctypes.c_ulonglong)
-        L.orc_scramble32_constants.restype = ctypes.POINTER(ctypes.c_uint)
-        L.orc_scramble64_constants.restype = ctypes.POINTER(ctypes.c_ulonglong)
+        L.orc_mtgp32_load.argtypes = [ctypes.c_char_p]
+        for name in ["orc_sobol32_vectors", "orc_scrambled_sobol32_vectors", "orc_scramble32_constants"]:
+            getattr(L, name).restype = ctypes.POINTER(ctypes.c_uint)
+        for name in ["orc_sobol64_vectors", "orc_scrambled_sobol64_vectors", "orc_scramble64_constants"]:
+            getattr(L, name).restype = ctypes.POINTER(ctypes.c_ulonglong)
+        # Published input tables shared with the library: Joe-Kuo direction numbers and the
+        # MTGP32 parameter sets (each derived by the scripts beside them).
+        if L.orc_sobol_load(os.path.join(DATA, "joe_kuo_20000.bin").encode()) != 0:
+            raise RuntimeError("oracle: cannot load the Joe-Kuo direction numbers")
+        if L.orc_mtgp32_load(os.path.join(DATA, "mtgp32_11213.bin").encode()) != 0:
+            raise RuntimeError("oracle: cannot load the MTGP32 parameter sets")
     return _lib
+
+
+def sobol_tables():
+    """(vectors32, scrambled32, vectors64, scrambled64, constants32, constants64) as numpy arrays."""
+    L = lib()
+    n32, n64 = 20000 * 32, 20000 * 64
+    return (np.ctypeslib.as_array(L.orc_sobol32_vectors(), (n32,)).copy(),
+            np.ctypeslib.as_array(L.orc_scrambled_sobol32_vectors(), (n32,)).copy(),
+            np.ctypeslib.as_array(L.orc_sobol64_vectors(), (n64,)).copy(),
+            np.ctypeslib.as_array(L.orc_scrambled_sobol64_vectors(), (n64,)).copy(),
+            np.ctypeslib.as_array(L.orc_scramble32_constants(), (20000,)).copy(),
+            np.ctypeslib.as_array(L.orc_scramble64_constants(), (20000,)).copy())
 
 
 def ref():
@@ -100,9 +120,29 @@ def ref():
                                    ctypes.c_uint64, ctypes.c_double, ctypes.c_double]
         R.ref_engine_stream.argtypes = [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_ulonglong,
                                         ctypes.c_ulonglong, ctypes.c_void_p, ctypes.c_size_t]
-        if hasattr(R, "ref_set_dimensions"):
-            R.ref_set_dimensions.argtypes = [ctypes.c_void_p, ctypes.c_uint]
+        R.ref_set_dimensions.argtypes = [ctypes.c_void_p, ctypes.c_uint]
+        R.ref_sobol64_set_vectors.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        for name in ["ref_sobol32_vectors", "ref_scrambled_sobol32_vectors", "ref_scramble32_constants"]:
+            getattr(R, name).restype = ctypes.POINTER(ctypes.c_uint)
+        R.ref_scramble64_constants.restype = ctypes.POINTER(ctypes.c_ulonglong)
+        # the reference tree has no 64-bit direction-vector tables: hand it the oracle's
+        t = sobol_tables()
+        _ref_keep.extend([t[2], t[3]])
+        R.ref_sobol64_set_vectors(t[2].ctypes.data, t[3].ctypes.data)
     return _ref
+
+
+_ref_keep = []
+
+
+def ref_sobol_tables():
+    """The reference's own (vectors32, scrambled32, constants32, constants64)."""
+    R = ref()
+    n32 = 20000 * 32
+    return (np.ctypeslib.as_array(R.ref_sobol32_vectors(), (n32,)).copy(),
+            np.ctypeslib.as_array(R.ref_scrambled_sobol32_vectors(), (n32,)).copy(),
+            np.ctypeslib.as_array(R.ref_scramble32_constants(), (20000,)).copy(),
+            np.ctypeslib.as_array(R.ref_scramble64_constants(), (20000,)).copy())
 
 
 class _Gen:

@@ -582,7 +582,7 @@ int orc_set_seed(orc_gen* g, unsigned long long seed)
     if(!g) return ST_NOT_CREATED;
     if(g->type == RNG_SOBOL32 || g->type == RNG_SOBOL64 || g->type == RNG_SCRAMBLED_SOBOL32
        || g->type == RNG_SCRAMBLED_SOBOL64)
-        return ST_TYPE_ERROR;
+        return ST_SUCCESS; /* sobol.hpp:581-584: the seed is ignored, nothing is reset */
     if(g->type == RNG_LFSR113)
     {
         /* lfsr113.hpp:240-255: only x and y are raised to their minimum seeds */
@@ -729,6 +729,59 @@ static void free_dist(dist* d)
 {
     free(d->pt.prob);
     free(d->pt.alias);
+}
+
+/* Distributions over 32-bit engine words for MTGP32 / MT19937 (orc_ext.c, orc_mt19937.c):
+   the same uniform/normal/log-normal/Poisson functors as Philox (mtgp32.hpp:375-470). */
+int orc_w32_dist(int kind, double a, double b, void** handle, unsigned* in_w, unsigned* out_w, unsigned* esz)
+{
+    orc_gen tmp;
+    memset(&tmp, 0, sizeof(tmp));
+    tmp.type = RNG_MTGP32;
+    dist* d  = (dist*)calloc(1, sizeof(dist));
+    const int st = make_dist(&tmp, kind, a, b, d);
+    if(st != 0)
+    {
+        free_dist(d);
+        free(d);
+        return st;
+    }
+    *handle = d;
+    *in_w   = d->in_w;
+    *out_w  = d->out_w;
+    *esz    = d->esz;
+    return 0;
+}
+void orc_w32_apply(void* handle, const uint64_t* in, uint8_t* out)
+{
+    const dist* d = (const dist*)handle;
+    d->apply(d, in, out);
+}
+void orc_w32_free(void* handle)
+{
+    free_dist((dist*)handle);
+    free(handle);
+}
+
+/* CDF table of the Poisson pmf (discrete.hpp:189-200 normalize, :276-286 running sum), used by
+   the quasi-random generators (sobol.hpp:500-503 select DISCRETE_METHOD_CDF). */
+int orc_poisson_cdf(double lambda, unsigned* size, unsigned* offset, double** cdf)
+{
+    double* p;
+    poisson_probs(lambda, &p, size, offset);
+    double sum = 0.0;
+    for(unsigned i = 0; i < *size; i++)
+        sum += p[i];
+    for(unsigned i = 0; i < *size; i++)
+        p[i] /= sum;
+    double run = 0.0;
+    for(unsigned i = 0; i < *size; i++)
+    {
+        run += p[i];
+        p[i] = run;
+    }
+    *cdf = p;
+    return 0;
 }
 
 /* philox4x32_10.hpp:106-214 generate_philox and threefry.hpp:86-198 generate_threefry. */

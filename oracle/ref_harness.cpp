@@ -5,7 +5,7 @@
 // second, independent checker for the oracle restatement: the engines, the
 // distribution functors and the generate_* kernel bodies executed here are the
 // reference's own code; this file only supplies the generator bookkeeping that
-// library/src/rng/{philox4x32_10,threefry,mrg,xorwow,lfsr113}.hpp perform around
+// library/src/rng/{philox4x32_10,threefry,mrg,xorwow,lfsr113,mtgp32,sobol}.hpp perform around
 // them (engine init, launch over the DEFAULT grid the way
 // library/src/rng/system.hpp:140-214 host_system::launch does, and the
 // post-call engine advance).  The public host generator cannot run here: its
@@ -23,7 +23,9 @@
 
 #include "rng/lfsr113.hpp"
 #include "rng/mrg.hpp"
+#include "rng/mtgp32.hpp"
 #include "rng/philox4x32_10.hpp"
+#include "rng/sobol.hpp"
 #include "rng/threefry.hpp"
 #include "rng/xorwow.hpp"
 
@@ -311,9 +313,214 @@ struct interleaved_gen : gen_base
     }
 };
 
+// ------------------------------------------------------------------- MTGP32
+// mtgp32.hpp:429-476 init (rocrand_make_state_mtgp32's host loop, rocrand_mtgp32.h:362-390,
+// without its hipMemcpy) and :478-521 generate: 512 blocks, one host thread per block.
+struct mtgp_gen : gen_base
+{
+    static constexpr unsigned G = 512; // config_defaults.hpp:45-50
+    std::vector<mtgp32_device_engine> engines;
+    using CP = default_config_provider<ROCRAND_RNG_PSEUDO_MTGP32>;
+
+    void init()
+    {
+        engines.assign(G, mtgp32_device_engine());
+        unsigned long long s = seed ^ (seed >> 32);
+        for(unsigned i = 0; i < G; i++)
+        {
+            auto& e = engines[i];
+            rocrand_device::rocrand_mtgp32_init_state(&e.m_state.status[0], &mtgp32dc_params_fast_11213[i],
+                                                      (unsigned int)s + i + 1);
+            e.m_state.offset = 0;
+            e.m_state.id     = i;
+            e.pos_tbl        = mtgp32dc_params_fast_11213[i].pos;
+            e.sh1_tbl        = mtgp32dc_params_fast_11213[i].sh1;
+            e.sh2_tbl        = mtgp32dc_params_fast_11213[i].sh2;
+            e.mask           = mtgp32dc_params_fast_11213[0].mask;
+            for(int j = 0; j < MTGP_TS; j++)
+            {
+                e.param_tbl[j]         = mtgp32dc_params_fast_11213[i].tbl[j];
+                e.temper_tbl[j]        = mtgp32dc_params_fast_11213[i].tmp_tbl[j];
+                e.single_temper_tbl[j] = mtgp32dc_params_fast_11213[i].flt_tmp_tbl[j];
+            }
+        }
+        initialized = true;
+    }
+
+    template<class T, class D>
+    int run(T* out, size_t n, uint64_t addr, D dist)
+    {
+        if(!initialized)
+            init();
+        aligned_out tmp(n * sizeof(T), addr);
+        T*          data = reinterpret_cast<T*>(tmp.p);
+        launch(G, 1, generate_mtgp<CP, false, T, D>, engines.data(), data, n, dist);
+        std::memcpy(out, data, n * sizeof(T));
+        return 0;
+    }
+
+    int generate(int kind, void* out, size_t n, uint64_t addr, double a, double b) override
+    {
+        switch(kind)
+        {
+            case K_U8: return run((unsigned char*)out, n, addr, uniform_distribution<unsigned char>());
+            case K_U16: return run((unsigned short*)out, n, addr, uniform_distribution<unsigned short>());
+            case K_U32: return run((unsigned int*)out, n, addr, uniform_distribution<unsigned int>());
+            case K_UF32: return run((float*)out, n, addr, uniform_distribution<float>());
+            case K_UF64: return run((double*)out, n, addr, uniform_distribution<double>());
+            case K_UF16: return run((__half*)out, n, addr, uniform_distribution<__half>());
+            case K_NF32: return run((float*)out, n, addr, normal_distribution<float>(float(a), float(b)));
+            case K_NF64: return run((double*)out, n, addr, normal_distribution<double>(a, b));
+            case K_NF16:
+                return run((__half*)out, n, addr, normal_distribution<__half>(__float2half(float(a)), __float2half(float(b))));
+            case K_LF32: return run((float*)out, n, addr, log_normal_distribution<float>(float(a), float(b)));
+            case K_LF64: return run((double*)out, n, addr, log_normal_distribution<double>(a, b));
+            case K_LF16:
+                return run((__half*)out, n, addr,
+                           log_normal_distribution<__half>(__float2half(float(a)), __float2half(float(b))));
+            case K_POISSON:
+            {
+                if(a <= 0.0)
+                    return ROCRAND_STATUS_OUT_OF_RANGE;
+                if(a > rocrand_device::detail::lambda_threshold_huge)
+                    return run((unsigned int*)out, n, addr, poisson_distribution_huge(a));
+                poisson_holder h(a);
+                return run((unsigned int*)out, n, addr, poisson_distribution<DISCRETE_METHOD_ALIAS>(h.st));
+            }
+        }
+        return ROCRAND_STATUS_TYPE_ERROR;
+    }
+};
+
+// -------------------------------------------------------------------- Sobol
+// 32-bit direction vectors come from the reference's own tables (library/src/*.S + .bin);
+// the 64-bit tables are absent from the reference tree and are handed in by the caller
+// (ref_sobol64_set_vectors).
+std::vector<unsigned long long> g_sobol64_vectors, g_scrambled_sobol64_vectors;
+
+struct cdf_holder
+{
+    rocrand_discrete_distribution_st st{};
+    std::vector<double>              cdf;
+    explicit cdf_holder(double lambda)
+    {
+        unsigned int        size, offset;
+        std::vector<double> p = calculate_poisson_probabilities(lambda, size, offset);
+        using F               = discrete_distribution_factory<DISCRETE_METHOD_CDF, true>;
+        F::normalize(p, size);
+        cdf.resize(size);
+        F::create_cdf(p, size, cdf.begin());
+        st.size   = size;
+        st.offset = offset;
+        st.cdf    = cdf.data();
+    }
+};
+
+template<bool Is64, bool Scrambled>
+struct sobol_gen : gen_base
+{
+    using constant_type = std::conditional_t<Is64, unsigned long long, unsigned int>;
+    using engine_type   = sobol_device_engine_t<Is64, Scrambled, false>;
+    unsigned dims           = 1;
+    unsigned current_offset = 0;
+
+    static const constant_type* vectors()
+    {
+        if constexpr(Is64)
+            return Scrambled ? g_scrambled_sobol64_vectors.data() : g_sobol64_vectors.data();
+        else
+            return Scrambled ? rocrand_h_scrambled_sobol32_direction_vectors
+                             : rocrand_h_sobol32_direction_vectors;
+    }
+
+    // sobol.hpp:618-718
+    template<class T, class D>
+    int run(T* out, size_t data_size, uint64_t addr, D dist)
+    {
+        if constexpr(!Is64 && std::is_same_v<T, unsigned long long>)
+            return ROCRAND_STATUS_TYPE_ERROR;
+        constexpr unsigned opt = sizeof(T) >= sizeof(int) ? 1 : sizeof(int) / sizeof(T);
+        if(data_size % dims != 0)
+            return ROCRAND_STATUS_LENGTH_NOT_MULTIPLE;
+        if(!initialized)
+        {
+            current_offset = static_cast<unsigned int>(offset);
+            initialized    = true;
+        }
+        const constant_type* v = vectors();
+        if(Is64 && v == nullptr)
+            return ROCRAND_STATUS_INTERNAL_ERROR;
+        // quirk of the generator's constructor (sobol.hpp:514): the scramble-constant
+        // pointer receives the direction-vector table
+        const constant_type* sc = v;
+        aligned_out        tmp(data_size * sizeof(T), addr);
+        T*                 data = reinterpret_cast<T*>(tmp.p);
+        const size_t       size = data_size / dims;
+        constexpr uint32_t threads = 256, max_blocks = 4096, opb = threads * opt;
+        const uint32_t blocks = std::min(max_blocks, static_cast<uint32_t>((size + opb - 1) / opb));
+        uint32_t       bx     = 1;
+        while(bx < (blocks + dims - 1) / dims)
+            bx *= 2;
+        for(uint32_t by = 0; by < dims; by++)
+            for(uint32_t x = 0; x < bx; x++)
+                for(uint32_t t = 0; t < threads; t++)
+                    generate_sobol_host<opt, Scrambled, engine_type, constant_type, T, D>(
+                        dim3(x, by), dim3(t), dim3(bx, dims), dim3(threads), data, size, v, sc,
+                        current_offset, dist);
+        std::memcpy(out, data, data_size * sizeof(T));
+        current_offset += size;
+        return 0;
+    }
+
+    int generate(int kind, void* out, size_t n, uint64_t addr, double a, double b) override
+    {
+        switch(kind)
+        {
+            case K_U8: return run((unsigned char*)out, n, addr, sobol_uniform_distribution<unsigned char>());
+            case K_U16: return run((unsigned short*)out, n, addr, sobol_uniform_distribution<unsigned short>());
+            case K_U32: return run((unsigned int*)out, n, addr, sobol_uniform_distribution<unsigned int>());
+            case K_U64:
+                if constexpr(Is64)
+                    return run((unsigned long long*)out, n, addr, sobol_uniform_distribution<unsigned long long>());
+                else
+                    return ROCRAND_STATUS_TYPE_ERROR;
+            case K_UF32: return run((float*)out, n, addr, sobol_uniform_distribution<float>());
+            case K_UF64: return run((double*)out, n, addr, sobol_uniform_distribution<double>());
+            case K_UF16: return run((__half*)out, n, addr, sobol_uniform_distribution<__half>());
+            case K_NF32: return run((float*)out, n, addr, sobol_normal_distribution<float>(float(a), float(b)));
+            case K_NF64: return run((double*)out, n, addr, sobol_normal_distribution<double>(a, b));
+            case K_NF16:
+                return run((__half*)out, n, addr,
+                           sobol_normal_distribution<__half>(__float2half(float(a)), __float2half(float(b))));
+            case K_LF32: return run((float*)out, n, addr, sobol_log_normal_distribution<float>(float(a), float(b)));
+            case K_LF64: return run((double*)out, n, addr, sobol_log_normal_distribution<double>(a, b));
+            case K_LF16:
+                return run((__half*)out, n, addr,
+                           sobol_log_normal_distribution<__half>(__float2half(float(a)), __float2half(float(b))));
+            case K_POISSON:
+            {
+                if(a <= 0.0)
+                    return ROCRAND_STATUS_OUT_OF_RANGE;
+                if(a > rocrand_device::detail::lambda_threshold_huge)
+                    return run((unsigned int*)out, n, addr, poisson_distribution_huge(a));
+                cdf_holder h(a);
+                return run((unsigned int*)out, n, addr, poisson_distribution<DISCRETE_METHOD_CDF>(h.st));
+            }
+        }
+        return ROCRAND_STATUS_TYPE_ERROR;
+    }
+};
+
 } // namespace
 
 extern "C" {
+
+static bool is_sobol(int t)
+{
+    return t == ROCRAND_RNG_QUASI_SOBOL32 || t == ROCRAND_RNG_QUASI_SCRAMBLED_SOBOL32
+           || t == ROCRAND_RNG_QUASI_SOBOL64 || t == ROCRAND_RNG_QUASI_SCRAMBLED_SOBOL64;
+}
+
 
 void* ref_create(int type)
 {
@@ -353,6 +560,11 @@ void* ref_create(int type)
         case ROCRAND_RNG_PSEUDO_LFSR113:
             g = new interleaved_gen<lfsr113_device_engine, ROCRAND_RNG_PSEUDO_LFSR113>();
             break;
+        case ROCRAND_RNG_PSEUDO_MTGP32: g = new mtgp_gen(); break;
+        case ROCRAND_RNG_QUASI_SOBOL32: g = new sobol_gen<false, false>(); break;
+        case ROCRAND_RNG_QUASI_SCRAMBLED_SOBOL32: g = new sobol_gen<false, true>(); break;
+        case ROCRAND_RNG_QUASI_SOBOL64: g = new sobol_gen<true, false>(); break;
+        case ROCRAND_RNG_QUASI_SCRAMBLED_SOBOL64: g = new sobol_gen<true, true>(); break;
         default: return nullptr;
     }
     g->type = type;
@@ -364,6 +576,8 @@ void ref_destroy(void* g) { delete static_cast<gen_base*>(g); }
 int ref_set_seed(void* p, unsigned long long seed)
 {
     auto* g = static_cast<gen_base*>(p);
+    if(is_sobol(g->type))
+        return 0; // sobol.hpp:581-584: ignored
     if(g->type == ROCRAND_RNG_PSEUDO_LFSR113)
     {
         // lfsr113.hpp:240-255
@@ -398,9 +612,41 @@ int ref_set_seed_uint4(void* p, const unsigned int* s4)
     return 0;
 }
 
+int ref_set_dimensions(void* p, unsigned int dims)
+{
+    auto* g = static_cast<gen_base*>(p);
+    if(!is_sobol(g->type))
+        return ROCRAND_STATUS_TYPE_ERROR;
+    if(dims < 1 || dims > 20000) // sobol.hpp:573-585
+        return ROCRAND_STATUS_OUT_OF_RANGE;
+    switch(g->type)
+    {
+        case ROCRAND_RNG_QUASI_SOBOL32: static_cast<sobol_gen<false, false>*>(g)->dims = dims; break;
+        case ROCRAND_RNG_QUASI_SCRAMBLED_SOBOL32: static_cast<sobol_gen<false, true>*>(g)->dims = dims; break;
+        case ROCRAND_RNG_QUASI_SOBOL64: static_cast<sobol_gen<true, false>*>(g)->dims = dims; break;
+        default: static_cast<sobol_gen<true, true>*>(g)->dims = dims; break;
+    }
+    g->initialized = false;
+    return 0;
+}
+
+// The 64-bit direction-vector tables (20000 x 64 words each), which the reference tree lacks.
+void ref_sobol64_set_vectors(const unsigned long long* plain, const unsigned long long* scrambled)
+{
+    g_sobol64_vectors.assign(plain, plain + 1280000);
+    g_scrambled_sobol64_vectors.assign(scrambled, scrambled + 1280000);
+}
+
+const unsigned int* ref_sobol32_vectors() { return rocrand_h_sobol32_direction_vectors; }
+const unsigned int* ref_scrambled_sobol32_vectors() { return rocrand_h_scrambled_sobol32_direction_vectors; }
+const unsigned int* ref_scramble32_constants() { return h_scrambled_sobol32_constants; }
+const unsigned long long* ref_scramble64_constants() { return h_scrambled_sobol64_constants; }
+
 int ref_set_offset(void* p, unsigned long long offset)
 {
     auto* g        = static_cast<gen_base*>(p);
+    if(g->type == ROCRAND_RNG_PSEUDO_MTGP32 || g->type == ROCRAND_RNG_PSEUDO_MT19937)
+        return ROCRAND_STATUS_TYPE_ERROR;
     g->offset      = offset;
     g->initialized = false;
     return 0;

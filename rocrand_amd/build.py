@@ -31,13 +31,21 @@ COMMON = [
 ]
 
 
+DATA = os.path.join(HERE, "data")
+
+
 def sources():
-    return sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
+    return sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp", ".S")))
 
 
 def _compile(src):
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *COMMON, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+    if src.endswith(".S"):
+        # read-only data tables (.incbin of rocrand_amd/data/*.bin)
+        cmd = ["gcc", "-c", "-fPIC", "-x", "assembler-with-cpp", f"-Wa,-I{DATA}",
+               os.path.join(CSRC, src), "-o", obj]
+    else:
+        cmd = [HIPCC, *COMMON, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {src}\n{r.stderr[-4000:]}")

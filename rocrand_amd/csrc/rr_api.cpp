@@ -115,6 +115,13 @@ unsigned int poisson_table(double lambda, unsigned int& offset, std::vector<doub
     return size;
 }
 
+// Scratch for the alias probabilities that the CDF method computes and drops.
+std::vector<double>& alias_unused_prob()
+{
+    thread_local std::vector<double> v;
+    return v;
+}
+
 constexpr double   LAMBDA_HUGE      = 4000.0; // rocrand_poisson.h:52-53
 constexpr unsigned POISSON_MAX_SIZE = 4096;   // >= table size at LAMBDA_HUGE (2086)
 
@@ -148,7 +155,13 @@ rocrand_status poisson_cache::get(double lambda, hipStream_t stream, dist_params
     {
         std::vector<double>       prob;
         std::vector<unsigned int> alias;
-        m_size = poisson_table(lambda, m_offset, prob, alias);
+        if(cdf_method)
+        {
+            std::vector<double> pmf = poisson_pmf(lambda, m_size, m_offset);
+            alias_table_from(pmf, m_size, alias_unused_prob(), alias, &prob);
+        }
+        else
+            m_size = poisson_table(lambda, m_offset, prob, alias);
         if(m_size > POISSON_MAX_SIZE)
             return ROCRAND_STATUS_INTERNAL_ERROR;
         // The staging buffers may still feed the previous table's copy.
@@ -165,8 +178,9 @@ rocrand_status poisson_cache::get(double lambda, hipStream_t stream, dist_params
     }
     p.pt.size   = m_size;
     p.pt.offset = m_offset;
-    p.pt.prob   = m_dprob;
+    p.pt.prob   = cdf_method ? nullptr : m_dprob;
     p.pt.alias  = m_dalias;
+    p.pt.cdf    = cdf_method ? m_dprob : nullptr;
     return ROCRAND_STATUS_SUCCESS;
 }
 

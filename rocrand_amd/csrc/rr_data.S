@@ -1,0 +1,24 @@
+/* rr_data.S -- published input tables linked into librocrand.so as read-only data.
+ *   joe_kuo_20000.bin : Joe-Kuo (new-joe-kuo-6.21201) Sobol direction numbers,
+ *                       dimensions 1..19999 (rocrand_amd/data/make_joe_kuo.py)
+ *   mtgp32_11213.bin  : the 512 MTGP32 parameter sets for exponent 11213
+ *                       (rocrand_amd/data/make_mtgp32_params.py)
+ * The build assembles this file with the data directory on the .incbin search path. */
+    .section .note.GNU-stack,"",@progbits
+    .section .rodata
+    .p2align 8
+    .globl rr_joe_kuo_begin
+    .hidden rr_joe_kuo_begin
+rr_joe_kuo_begin:
+    .incbin "joe_kuo_20000.bin"
+    .globl rr_joe_kuo_end
+    .hidden rr_joe_kuo_end
+rr_joe_kuo_end:
+    .p2align 8
+    .globl rr_mtgp32_params_begin
+    .hidden rr_mtgp32_params_begin
+rr_mtgp32_params_begin:
+    .incbin "mtgp32_11213.bin"
+    .globl rr_mtgp32_params_end
+    .hidden rr_mtgp32_params_end
+rr_mtgp32_params_end:

@@ -197,6 +197,7 @@ struct alias_table
     unsigned int        offset;
     const unsigned int* alias;
     const double*       prob;
+    const double*       cdf; // cumulative table (quasi-random generators)
 };
 
 __device__ __forceinline__ unsigned int discrete_alias(double x, const alias_table& t)

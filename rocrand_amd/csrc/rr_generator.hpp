@@ -42,11 +42,14 @@ public:
     ~poisson_cache();
     // Fills p for lambda (<= 4000: table method, else the normal approximation).
     rocrand_status get(double lambda, hipStream_t stream, dist_params& p);
+    // Quasi-random generators sample by binary search in the cumulative table
+    // (sobol.hpp:500-503 selects DISCRETE_METHOD_CDF) instead of the alias table.
+    bool cdf_method = false;
 
 private:
     double        m_lambda = -1.0;
     unsigned int  m_size = 0, m_offset = 0;
-    double*       m_dprob  = nullptr;
+    double*       m_dprob  = nullptr; // alias probabilities, or the CDF
     unsigned int* m_dalias = nullptr;
     double*       m_hprob  = nullptr; // pinned staging
     unsigned int* m_halias = nullptr;

@@ -71,6 +71,34 @@ hipError_t launch_interleaved_init(inter_kind k,
                                    hipStream_t s);
 hipError_t launch_interleaved(inter_kind k, out_kind o, const interleaved_call& c, hipStream_t s);
 
+// One Sobol generate call (rr_sobol.hip).
+struct sobol_call
+{
+    const void* vectors;  // device direction vectors, [dims][32 or 64]
+    const void* scramble; // scramble constant table indexed by dimension, or nullptr
+    void*       data;
+    size_t      size;     // points per dimension
+    uint32_t    offset;   // index of the first point
+    uint32_t    dims;
+    // filled by launch_sobol
+    uint64_t    first_group;
+    uint32_t    groups_per_wave;
+    uint32_t    waves;
+    dist_params params;
+};
+hipError_t launch_sobol(bool is64, out_kind o, sobol_call c, hipStream_t s);
+
+// Sobol tables (rr_quasi.cpp): host arrays of the reference's layout, and the
+// device copies of the current device.
+struct sobol_device_tables
+{
+    const uint32_t* v32;
+    const uint32_t* sv32; // scrambled
+    const uint64_t* v64;
+    const uint64_t* sv64;
+};
+hipError_t get_sobol_device_tables(sobol_device_tables& t);
+
 // Host-computed jump tables, uploaded once per device (rr_tables.cpp).
 struct device_tables
 {

@@ -1,0 +1,215 @@
+"""GPU parity of the Sobol32/64 (plain, scrambled) and MTGP32 generators against the
+oracle, through the C ABI on cuda:0.
+
+Bar: integers, uniforms and Poisson (CDF search for Sobol) bit-exact.  Sobol
+normal/log-normal use the reference's closed-form inverse error function
+(rocrand_normal.h:174-296), -tt1 + sqrt(tt1^2 - tt2) with tt1 ~ 4.33: near the
+median that difference cancels, so a one-ulp difference in the logarithm (the host
+libm's float logf is not correctly rounded for ~0.03% of these inputs; the device
+rounds correctly) can move tt1's rounding by one ulp, which the final square root
+turns into an output change of up to ~ulp(tt1)/|v|.  The check therefore asks for
+bit equality on almost every value and, for the rest, a difference within
+3 ulp(v) + 4 ulp(4.33)/|v|, the propagated rounding of tt1.  MTGP32 normals use the
+Box-Muller functors of the pseudo-random engines and get the same bar as those.
+"""
+import numpy as np
+import pytest
+
+from test_gpu_parity import KINDS, _check
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as O  # noqa: E402
+from rocrand_amd import rocrand as R  # noqa: E402
+
+SOBOL = {"sobol32": R.SOBOL32, "scrambled_sobol32": R.SCRAMBLED_SOBOL32,
+         "sobol64": R.SOBOL64, "scrambled_sobol64": R.SCRAMBLED_SOBOL64}
+
+# (dims, offset, points per dimension, element shift of the output pointer)
+SOBOL_CONFIGS = [(1, 0, 1000, 0), (3, 1000, 1001, 1), (7, 5, 37, 3), (64, 123457, 4099, 2),
+                 (1, (1 << 31) - 100, 1000, 1)]
+
+
+def _sobol_gpu(rng, kind, dims, offset, npd, shift, calls=2, args=()):
+    dt, meth, a = KINDS[kind]
+    if args:
+        a = args
+    g = R.QRNG(rng, ndim=dims, offset=offset)
+    n = dims * npd
+    out = []
+    for _ in range(calls):
+        buf = torch.zeros(n + 16, dtype=dt, device="cuda")
+        view = buf[shift:shift + n]
+        getattr(g, meth)(view, *a)
+        torch.cuda.synchronize()
+        out.append((view.cpu().numpy().copy(), view.data_ptr()))
+    return out
+
+
+def _check_erfinv(kind, got, want, mean, sd):
+    """Sobol normal / log-normal bar (see the module docstring)."""
+    assert got.shape == want.shape
+    single = got.dtype != np.float64
+    c1_ulp = float(np.spacing(np.float32(4.33))) if single else float(np.spacing(4.33))
+    if kind.startswith("log"):
+        zg, zw = np.log(got.astype(np.float64)), np.log(want.astype(np.float64))
+    else:
+        zg, zw = got.astype(np.float64), want.astype(np.float64)
+    v = (zw - mean) / sd
+    ftype = np.float32 if single else np.float64
+    r = np.maximum(np.abs(v) / np.sqrt(2.0), 1e-30)
+    bound = (3 * np.spacing(np.abs(v).astype(ftype)).astype(np.float64)
+             + 4 * c1_ulp * np.sqrt(2.0) / (2 * r)) * abs(sd)
+    slack = np.spacing(np.abs(zw).astype(ftype)).astype(np.float64) * 2
+    if got.dtype == np.float16:
+        slack = slack + np.spacing(np.abs(zw).astype(np.float16)).astype(np.float64) * 2
+    err = np.abs(zg - zw)
+    bad = np.nonzero(~(err <= bound + slack))[0]
+    assert bad.size == 0, f"{bad.size} values beyond the bound, first {bad[:3]}: got {got[bad[:3]]} want {want[bad[:3]]}"
+    same = np.count_nonzero(got.view(np.uint8 if got.dtype.itemsize == 1 else f"u{got.dtype.itemsize}")
+                            == want.view(np.uint8 if want.dtype.itemsize == 1 else f"u{want.dtype.itemsize}"))
+    assert same >= 0.99 * got.size, f"only {same} of {got.size} bit-equal"
+
+
+def _sobol_cases():
+    for name, rng in SOBOL.items():
+        for kind in KINDS:
+            if kind == "u64" and "64" not in name:
+                continue
+            yield name, kind
+
+
+@pytest.mark.parametrize("name,kind", list(_sobol_cases()))
+def test_sobol_parity(name, kind):
+    rng = SOBOL[name]
+    a = KINDS[kind][2]
+    a0 = a[0] if len(a) > 0 else 0.0
+    a1 = a[1] if len(a) > 1 else 0.0
+    for dims, offset, npd, shift in SOBOL_CONFIGS:
+        print(f"case {name} {kind} dims={dims} offset={offset} points={npd} shift={shift}", flush=True)
+        got = _sobol_gpu(rng, kind, dims, offset, npd, shift)
+        og = O.OracleGen(rng, dims=dims, offset=offset)
+        for g, addr in got:
+            st, want = og.generate(kind, dims * npd, addr, a0, a1)
+            assert st == 0
+            if "normal" in kind:
+                _check_erfinv(kind, g, want, a0, a1)
+            else:
+                _check(kind, g, want, (a0, a1))
+
+
+@pytest.mark.parametrize("name", list(SOBOL))
+@pytest.mark.parametrize("lam", [0.5, 4.0, 1000.0, 5000.0])
+def test_sobol_poisson_cdf(name, lam):
+    rng = SOBOL[name]
+    got = _sobol_gpu(rng, "poisson", 4, 77, 5003, 1, args=(lam,))
+    og = O.OracleGen(rng, dims=4, offset=77)
+    for g, addr in got:
+        st, want = og.generate("poisson", 4 * 5003, addr, lam, 0.0)
+        assert st == 0
+        if lam > 4000.0:
+            assert np.max(np.abs(g.astype(np.int64) - want.astype(np.int64))) <= 1
+        else:
+            assert np.array_equal(g, want)
+
+
+def test_sobol_length_not_multiple_and_type_error():
+    g = R.QRNG(R.SOBOL32, ndim=3)
+    buf = torch.zeros(10, dtype=torch.uint32, device="cuda")
+    with pytest.raises(R.RocRandError) as e:
+        g.generate(buf)
+    assert e.value.value == 105
+    b64 = torch.zeros(9, dtype=torch.uint64, device="cuda")
+    with pytest.raises(R.RocRandError) as e:
+        g.generate(b64)
+    assert e.value.value == 103
+
+
+@pytest.mark.parametrize("name,dims", [("sobol32", 1000), ("scrambled_sobol64", 16)])
+def test_sobol_full_size_sampled(name, dims):
+    """dims x 2^20 points (the quasi-random bench shape): sampled positions against the
+    closed form x = XOR of the oracle's direction vectors over Gray(offset + p)."""
+    rng = SOBOL[name]
+    is64 = "64" in name
+    npd, offset = 1 << 20, 1000
+    dt = torch.uint64 if is64 else torch.uint32
+    out = torch.empty(dims * npd, dtype=dt, device="cuda")
+    g = R.QRNG(rng, ndim=dims, offset=offset)
+    g.generate(out)
+    torch.cuda.synchronize()
+    v32, sv32, v64, sv64, _, _ = O.sobol_tables()
+    vec = (sv64 if "scrambled" in name else v64) if is64 else (sv32 if "scrambled" in name else v32)
+    B = 64 if is64 else 32
+    vec = vec.reshape(20000, B)
+    rs = np.random.default_rng(3)
+    d = rs.integers(0, dims, 1 << 15)
+    p = rs.integers(0, npd, 1 << 15)
+    p[:dims] = 0
+    d[:dims] = np.arange(dims)
+    i = (p + offset).astype(np.uint64)
+    gray = i ^ (i >> np.uint64(1))
+    want = np.zeros(p.size, dtype=np.uint64)
+    for k in range(32):
+        bit = ((gray >> np.uint64(k)) & np.uint64(1)).astype(bool)
+        want ^= np.where(bit, vec[d, k].astype(np.uint64), np.uint64(0))
+    if "scrambled" in name:
+        want ^= vec.reshape(-1)[d].astype(np.uint64)
+    idx = torch.from_numpy((d * npd + p).astype(np.int64)).cuda()
+    signed = out.view(torch.int64 if is64 else torch.int32)
+    got = signed[idx].cpu().numpy().view(np.uint64 if is64 else np.uint32).astype(np.uint64)
+    assert np.array_equal(got, want)
+
+
+# ------------------------------------------------------------------ MTGP32
+MTGP_CONFIGS = [(12345, 1000, 0), (0, 131072 + 777, 1), (0xFEDCBA9876543210, 13, 3), (7, 262144 * 2 + 5, 2)]
+
+
+def _mtgp_cases():
+    for kind in KINDS:
+        if kind != "u64":
+            yield kind
+
+
+@pytest.mark.parametrize("kind", list(_mtgp_cases()))
+def test_mtgp32_parity(kind):
+    dt, meth, a = KINDS[kind]
+    a0 = a[0] if len(a) > 0 else 0.0
+    a1 = a[1] if len(a) > 1 else 0.0
+    for seed, n, shift in MTGP_CONFIGS:
+        print(f"case mtgp32 {kind} seed={seed:#x} n={n} shift={shift}", flush=True)
+        g = R.PRNG(R.MTGP32, seed=seed)
+        og = O.OracleGen(O.MTGP32, seed=seed)
+        for _ in range(2):
+            buf = torch.zeros(n + 16, dtype=dt, device="cuda")
+            view = buf[shift:shift + n]
+            getattr(g, meth)(view, *a)
+            torch.cuda.synchronize()
+            st, want = og.generate(kind, n, view.data_ptr(), a0, a1)
+            assert st == 0
+            _check(kind, view.cpu().numpy(), want, (a0, a1))
+
+
+def test_mtgp32_large():
+    """2^26 u32 then 2^26 uniform floats on one MTGP32 generator, bit-exact."""
+    n = 1 << 26
+    g = R.PRNG(R.MTGP32, seed=12345)
+    a = torch.empty(n, dtype=torch.uint32, device="cuda")
+    b = torch.empty(n, dtype=torch.float32, device="cuda")
+    g.generate(a)
+    g.uniform(b)
+    torch.cuda.synchronize()
+    og = O.OracleGen(O.MTGP32, seed=12345)
+    st, wa = og.generate("u32", n, a.data_ptr())
+    st2, wb = og.generate("uniform_f32", n, b.data_ptr())
+    assert st == st2 == 0
+    assert np.array_equal(a.cpu().numpy(), wa)
+    assert np.array_equal(b.cpu().numpy(), wb)
+
+
+def test_mtgp32_set_offset_is_type_error():
+    g = R.PRNG(R.MTGP32)
+    with pytest.raises(R.RocRandError) as e:
+        g.offset = 5
+    assert e.value.value == 103
