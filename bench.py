@@ -60,10 +60,12 @@ def per_generator_table(R, torch, stream, n, steps):
              ("normal_f64", torch.float64, lambda g, b: g.normal(b, 0.0, 1.0)),
              ("lognormal_f32", torch.float32, lambda g, b: g.lognormal(b, 0.0, 1.0)),
              ("poisson_10", torch.uint32, lambda g, b: g.poisson(b, 10.0))]
+    engines += [("mtgp32", R.MTGP32), ("sobol32", R.SOBOL32), ("scrambled_sobol32", R.SCRAMBLED_SOBOL32),
+                ("sobol64", R.SOBOL64), ("scrambled_sobol64", R.SCRAMBLED_SOBOL64)]
     table = {}
     bufs = {}
     for ename, rng in engines:
-        g = R.PRNG(rng, seed=12345)
+        g = R.QRNG(rng) if "sobol" in ename else R.PRNG(rng, seed=12345)
         g.stream = stream
         row = {}
         for dname, dt, call in dists:
@@ -84,6 +86,26 @@ def store_ceiling(torch, stream, n, steps):
     with torch.cuda.stream(stream):
         ms = time_calls(lambda: buf.fill_(1.0), stream, steps, 3)
     return _gbs(n * 4, ms)
+
+
+def copy_calibration(torch, n):
+    """A plain device copy of n floats (n*4 bytes read, n*4 written): the known-size kernel
+    whose FETCH_SIZE / WRITE_SIZE counters calibrate the generator kernel's HBM traffic."""
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+
+
+def load_traffic():
+    """HBM bytes per launch of the headline kernel, from the committed counter summary."""
+    path = os.path.join(HERE, "profiles", "traffic_philox_uniform_f32_2e28.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("traffic_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
 
 
 def cpu_baseline(n):
@@ -109,12 +131,15 @@ def cpu_baseline(n):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=200)
+    # ~0.5 s of untimed calls: the first calls after process start run at lower clocks
+    ap.add_argument("--warmup", type=int, default=3000)
     ap.add_argument("--n", type=int, default=1 << 28)
     ap.add_argument("--no-table", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=1 << 28)
+    ap.add_argument("--calib", action="store_true",
+                    help="also run a plain 2^28-float device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every generator here runs on a single MI355X and the "
@@ -125,6 +150,8 @@ def main():
     from rocrand_amd import rocrand as R
 
     n = args.n
+    if args.calib:
+        copy_calibration(torch, 1 << 28)
     stream = torch.cuda.Stream()
     out = torch.empty(n, dtype=torch.float32, device="cuda")
     g = R.PRNG(R.PHILOX4_32_10, seed=12345)
@@ -171,7 +198,7 @@ def main():
         "config": {"workload": "plain bulk path, Philox4x32-10 uniform fp32, 2^28 samples, seed 12345, offset 0",
                    "n": n, "samples_per_s": round(n / (ms * 1e-3), 1)},
         "roofline": {"bound": "hbm", "achieved": round(value, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(value / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(value / HBM_PEAK_GBS, 4), "traffic": load_traffic(),
                      "measured_store_ceiling_gbs": round(ceiling, 1)},
         "cpu_baseline": cpu,
         "per_generator": table,

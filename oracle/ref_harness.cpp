@@ -23,6 +23,7 @@
 
 #include "rng/lfsr113.hpp"
 #include "rng/mrg.hpp"
+#include "rng/mt19937.hpp"
 #include "rng/mtgp32.hpp"
 #include "rng/philox4x32_10.hpp"
 #include "rng/sobol.hpp"
@@ -313,6 +314,41 @@ struct interleaved_gen : gen_base
     }
 };
 
+// The generic 32-bit word distributions (uniform.hpp, normal.hpp, log_normal.hpp, poisson.hpp
+// with the alias method) as used by the MTGP32 and MT19937 generators.
+template<class G>
+int dispatch_w32(G& g, int kind, void* out, size_t n, uint64_t addr, double a, double b)
+{
+    switch(kind)
+    {
+        case K_U8: return g.run((unsigned char*)out, n, addr, uniform_distribution<unsigned char>());
+        case K_U16: return g.run((unsigned short*)out, n, addr, uniform_distribution<unsigned short>());
+        case K_U32: return g.run((unsigned int*)out, n, addr, uniform_distribution<unsigned int>());
+        case K_UF32: return g.run((float*)out, n, addr, uniform_distribution<float>());
+        case K_UF64: return g.run((double*)out, n, addr, uniform_distribution<double>());
+        case K_UF16: return g.run((__half*)out, n, addr, uniform_distribution<__half>());
+        case K_NF32: return g.run((float*)out, n, addr, normal_distribution<float>(float(a), float(b)));
+        case K_NF64: return g.run((double*)out, n, addr, normal_distribution<double>(a, b));
+        case K_NF16:
+            return g.run((__half*)out, n, addr, normal_distribution<__half>(__float2half(float(a)), __float2half(float(b))));
+        case K_LF32: return g.run((float*)out, n, addr, log_normal_distribution<float>(float(a), float(b)));
+        case K_LF64: return g.run((double*)out, n, addr, log_normal_distribution<double>(a, b));
+        case K_LF16:
+            return g.run((__half*)out, n, addr,
+                         log_normal_distribution<__half>(__float2half(float(a)), __float2half(float(b))));
+        case K_POISSON:
+        {
+            if(a <= 0.0)
+                return ROCRAND_STATUS_OUT_OF_RANGE;
+            if(a > rocrand_device::detail::lambda_threshold_huge)
+                return g.run((unsigned int*)out, n, addr, poisson_distribution_huge(a));
+            poisson_holder h(a);
+            return g.run((unsigned int*)out, n, addr, poisson_distribution<DISCRETE_METHOD_ALIAS>(h.st));
+        }
+    }
+    return ROCRAND_STATUS_TYPE_ERROR;
+}
+
 // ------------------------------------------------------------------- MTGP32
 // mtgp32.hpp:429-476 init (rocrand_make_state_mtgp32's host loop, rocrand_mtgp32.h:362-390,
 // without its hipMemcpy) and :478-521 generate: 512 blocks, one host thread per block.
@@ -361,34 +397,77 @@ struct mtgp_gen : gen_base
 
     int generate(int kind, void* out, size_t n, uint64_t addr, double a, double b) override
     {
-        switch(kind)
+        return dispatch_w32(*this, kind, out, n, addr, a, b);
+    }
+};
+
+// ------------------------------------------------------------------ MT19937
+// mt19937.hpp:793-908 init (jump_ahead_mt19937, init_engines_mt19937) and :911-1042
+// generate (short/long kernel choice and the start-input bookkeeping), over the default
+// grid 256 x 256 (config_defaults.hpp:76-80: 8192 generators of 8 threads).  The jump
+// kernel's host form costs ~10 ms per generator, so only the first `limit` generators
+// are set up (the others stay zero): callers keep every touched thread below 8*limit.
+// The host form of the kernels shares one head/tail flag among eight emulated lanes
+// (mt19937.hpp:355-431), so only aligned calls of whole vectors are comparable.
+struct mt_gen : gen_base
+{
+    static constexpr unsigned G = 8192, STRIDE = 65536, NN = mt19937_constants::n;
+    using CP = default_config_provider<ROCRAND_RNG_PSEUDO_MT19937>;
+    unsigned              limit = 64;
+    std::vector<unsigned> engines;
+    unsigned              start_input = 0, prev_in_w = 0;
+
+    void init()
+    {
+        std::vector<unsigned> base(size_t(G) * NN, 0u);
+        launch(limit, 128, jump_ahead_mt19937<128, CP, false>, base.data(), seed, rocrand_h_mt19937_jump);
+        engines.assign(size_t(NN / 8) * STRIDE, 0u);
+        launch(256, 256, init_engines_mt19937<CP, false>, engines.data(), (const unsigned*)base.data());
+        start_input = 0;
+        prev_in_w   = 0;
+        initialized = true;
+    }
+
+    template<class T, class D>
+    int run(T* out, size_t size, uint64_t addr, D dist)
+    {
+        if(!initialized)
+            init();
+        constexpr unsigned in_w = D::input_width, out_w = D::output_width;
+        constexpr unsigned ips  = (NN / 8) / in_w;
+        const unsigned     full_stride = STRIDE * ips;
+        using vec_type     = rocrand_impl::aligned_vec_type<T, out_w>;
+        aligned_out tmp(size * sizeof(T), addr);
+        T*          data = reinterpret_cast<T*>(tmp.p);
+        const uintptr_t    up    = reinterpret_cast<uintptr_t>(data);
+        const size_t       mis   = (out_w - up / sizeof(T) % out_w) % out_w;
+        const unsigned int head  = static_cast<unsigned>(std::min<size_t>(size, mis));
+        const unsigned int tail  = static_cast<unsigned>((size - head) % out_w);
+        const size_t       vec_size = (size - head) / out_w;
+        const unsigned     extra = (head > 0 || tail > 0) ? 1 : 0;
+        vec_type*          vec_data = reinterpret_cast<vec_type*>(data + mis);
+        if(prev_in_w != in_w && start_input > 0)
         {
-            case K_U8: return run((unsigned char*)out, n, addr, uniform_distribution<unsigned char>());
-            case K_U16: return run((unsigned short*)out, n, addr, uniform_distribution<unsigned short>());
-            case K_U32: return run((unsigned int*)out, n, addr, uniform_distribution<unsigned int>());
-            case K_UF32: return run((float*)out, n, addr, uniform_distribution<float>());
-            case K_UF64: return run((double*)out, n, addr, uniform_distribution<double>());
-            case K_UF16: return run((__half*)out, n, addr, uniform_distribution<__half>());
-            case K_NF32: return run((float*)out, n, addr, normal_distribution<float>(float(a), float(b)));
-            case K_NF64: return run((double*)out, n, addr, normal_distribution<double>(a, b));
-            case K_NF16:
-                return run((__half*)out, n, addr, normal_distribution<__half>(__float2half(float(a)), __float2half(float(b))));
-            case K_LF32: return run((float*)out, n, addr, log_normal_distribution<float>(float(a), float(b)));
-            case K_LF64: return run((double*)out, n, addr, log_normal_distribution<double>(a, b));
-            case K_LF16:
-                return run((__half*)out, n, addr,
-                           log_normal_distribution<__half>(__float2half(float(a)), __float2half(float(b))));
-            case K_POISSON:
-            {
-                if(a <= 0.0)
-                    return ROCRAND_STATUS_OUT_OF_RANGE;
-                if(a > rocrand_device::detail::lambda_threshold_huge)
-                    return run((unsigned int*)out, n, addr, poisson_distribution_huge(a));
-                poisson_holder h(a);
-                return run((unsigned int*)out, n, addr, poisson_distribution<DISCRETE_METHOD_ALIAS>(h.st));
-            }
+            const unsigned used = (start_input + STRIDE - 1) / STRIDE * prev_in_w;
+            start_input         = (used + in_w - 1) / in_w * STRIDE;
+            if(start_input >= full_stride)
+                start_input = 0;
         }
-        return ROCRAND_STATUS_TYPE_ERROR;
+        if(start_input > 0 && start_input + vec_size + extra <= full_stride)
+            launch(256, 256, generate_short_mt19937<CP, false, T, vec_type, D>, engines.data(), start_input, data,
+                   size, vec_data, vec_size, head, tail, dist);
+        else
+            launch(256, 256, generate_long_mt19937<CP, false, T, vec_type, D>, engines.data(), start_input, data,
+                   size, vec_data, vec_size, head, tail, dist);
+        std::memcpy(out, data, size * sizeof(T));
+        start_input = static_cast<unsigned>((start_input + vec_size + extra) % full_stride);
+        prev_in_w   = in_w;
+        return 0;
+    }
+
+    int generate(int kind, void* out, size_t n, uint64_t addr, double a, double b) override
+    {
+        return dispatch_w32(*this, kind, out, n, addr, a, b);
     }
 };
 
@@ -561,6 +640,7 @@ void* ref_create(int type)
             g = new interleaved_gen<lfsr113_device_engine, ROCRAND_RNG_PSEUDO_LFSR113>();
             break;
         case ROCRAND_RNG_PSEUDO_MTGP32: g = new mtgp_gen(); break;
+        case ROCRAND_RNG_PSEUDO_MT19937: g = new mt_gen(); break;
         case ROCRAND_RNG_QUASI_SOBOL32: g = new sobol_gen<false, false>(); break;
         case ROCRAND_RNG_QUASI_SCRAMBLED_SOBOL32: g = new sobol_gen<false, true>(); break;
         case ROCRAND_RNG_QUASI_SOBOL64: g = new sobol_gen<true, false>(); break;
@@ -638,6 +718,18 @@ void ref_sobol64_set_vectors(const unsigned long long* plain, const unsigned lon
 }
 
 const unsigned int* ref_sobol32_vectors() { return rocrand_h_sobol32_direction_vectors; }
+const unsigned int* ref_mt19937_jump() { return rocrand_h_mt19937_jump; }
+
+// MT19937: number of generators whose engines the next initialisation sets up.
+int ref_mt19937_set_limit(void* p, unsigned int limit)
+{
+    auto* g = static_cast<gen_base*>(p);
+    if(g->type != ROCRAND_RNG_PSEUDO_MT19937 || limit < 1 || limit > 8192)
+        return ROCRAND_STATUS_OUT_OF_RANGE;
+    static_cast<mt_gen*>(g)->limit = limit;
+    g->initialized                 = false;
+    return 0;
+}
 const unsigned int* ref_scrambled_sobol32_vectors() { return rocrand_h_scrambled_sobol32_direction_vectors; }
 const unsigned int* ref_scramble32_constants() { return h_scrambled_sobol32_constants; }
 const unsigned long long* ref_scramble64_constants() { return h_scrambled_sobol64_constants; }

@@ -1,20 +1,14 @@
 """GPU parity of the Sobol32/64 (plain, scrambled) and MTGP32 generators against the
 oracle, through the C ABI on cuda:0.
 
-Bar: integers, uniforms and Poisson (CDF search for Sobol) bit-exact.  Sobol
-normal/log-normal use the reference's closed-form inverse error function
-(rocrand_normal.h:174-296), -tt1 + sqrt(tt1^2 - tt2) with tt1 ~ 4.33: near the
-median that difference cancels, so a one-ulp difference in the logarithm (the host
-libm's float logf is not correctly rounded for ~0.03% of these inputs; the device
-rounds correctly) can move tt1's rounding by one ulp, which the final square root
-turns into an output change of up to ~ulp(tt1)/|v|.  The check therefore asks for
-bit equality on almost every value and, for the rest, a difference within
-3 ulp(v) + 4 ulp(4.33)/|v|, the propagated rounding of tt1.  MTGP32 normals use the
-Box-Muller functors of the pseudo-random engines and get the same bar as those.
+Bar: tests/parity_checks.py.  Sobol normals go through the reference's inverse error
+function (rocrand_normal.h:174-296), judged with the bar for that formula; MTGP32 uses
+the Box-Muller functors of the pseudo-random engines.
 """
 import numpy as np
 import pytest
 
+from parity_checks import check
 from test_gpu_parity import KINDS, _check
 
 pytestmark = pytest.mark.gpu
@@ -48,31 +42,6 @@ def _sobol_gpu(rng, kind, dims, offset, npd, shift, calls=2, args=()):
     return out
 
 
-def _check_erfinv(kind, got, want, mean, sd):
-    """Sobol normal / log-normal bar (see the module docstring)."""
-    assert got.shape == want.shape
-    single = got.dtype != np.float64
-    c1_ulp = float(np.spacing(np.float32(4.33))) if single else float(np.spacing(4.33))
-    if kind.startswith("log"):
-        zg, zw = np.log(got.astype(np.float64)), np.log(want.astype(np.float64))
-    else:
-        zg, zw = got.astype(np.float64), want.astype(np.float64)
-    v = (zw - mean) / sd
-    ftype = np.float32 if single else np.float64
-    r = np.maximum(np.abs(v) / np.sqrt(2.0), 1e-30)
-    bound = (3 * np.spacing(np.abs(v).astype(ftype)).astype(np.float64)
-             + 4 * c1_ulp * np.sqrt(2.0) / (2 * r)) * abs(sd)
-    slack = np.spacing(np.abs(zw).astype(ftype)).astype(np.float64) * 2
-    if got.dtype == np.float16:
-        slack = slack + np.spacing(np.abs(zw).astype(np.float16)).astype(np.float64) * 2
-    err = np.abs(zg - zw)
-    bad = np.nonzero(~(err <= bound + slack))[0]
-    assert bad.size == 0, f"{bad.size} values beyond the bound, first {bad[:3]}: got {got[bad[:3]]} want {want[bad[:3]]}"
-    same = np.count_nonzero(got.view(np.uint8 if got.dtype.itemsize == 1 else f"u{got.dtype.itemsize}")
-                            == want.view(np.uint8 if want.dtype.itemsize == 1 else f"u{want.dtype.itemsize}"))
-    assert same >= 0.99 * got.size, f"only {same} of {got.size} bit-equal"
-
-
 def _sobol_cases():
     for name, rng in SOBOL.items():
         for kind in KINDS:
@@ -95,7 +64,7 @@ def test_sobol_parity(name, kind):
             st, want = og.generate(kind, dims * npd, addr, a0, a1)
             assert st == 0
             if "normal" in kind:
-                _check_erfinv(kind, g, want, a0, a1)
+                check(kind, g, want, (a0, a1), erfinv=True)
             else:
                 _check(kind, g, want, (a0, a1))
 
