@@ -60,7 +60,7 @@ def per_generator_table(R, torch, stream, n, steps):
              ("normal_f64", torch.float64, lambda g, b: g.normal(b, 0.0, 1.0)),
              ("lognormal_f32", torch.float32, lambda g, b: g.lognormal(b, 0.0, 1.0)),
              ("poisson_10", torch.uint32, lambda g, b: g.poisson(b, 10.0))]
-    engines += [("mtgp32", R.MTGP32), ("sobol32", R.SOBOL32), ("scrambled_sobol32", R.SCRAMBLED_SOBOL32),
+    engines += [("mtgp32", R.MTGP32), ("mt19937", R.MT19937), ("sobol32", R.SOBOL32), ("scrambled_sobol32", R.SCRAMBLED_SOBOL32),
                 ("sobol64", R.SOBOL64), ("scrambled_sobol64", R.SCRAMBLED_SOBOL64)]
     table = {}
     bufs = {}

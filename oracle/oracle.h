@@ -106,6 +106,9 @@ const unsigned long long* orc_scramble64_constants(void);
 /* MTGP32 parameter sets (512 x {pos, sh1, sh2, tbl[16], tmp_tbl[16], mask}). */
 int orc_mtgp32_load(const char* params_path);
 
+/* MT19937 jump polynomials (510 x 624 words, see rocrand_amd/data/make_mt19937_jumps.c). */
+int orc_mt19937_load(const char* jumps_path);
+
 #ifdef __cplusplus
 }
 #endif

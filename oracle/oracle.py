@@ -87,6 +87,9 @@ def lib():
             raise RuntimeError("oracle: cannot load the Joe-Kuo direction numbers")
         if L.orc_mtgp32_load(os.path.join(DATA, "mtgp32_11213.bin").encode()) != 0:
             raise RuntimeError("oracle: cannot load the MTGP32 parameter sets")
+        L.orc_mt19937_load.argtypes = [ctypes.c_char_p]
+        if L.orc_mt19937_load(os.path.join(DATA, "mt19937_jumps.bin").encode()) != 0:
+            raise RuntimeError("oracle: cannot load the MT19937 jump polynomials")
     return _lib
 
 
@@ -125,6 +128,8 @@ def ref():
         for name in ["ref_sobol32_vectors", "ref_scrambled_sobol32_vectors", "ref_scramble32_constants"]:
             getattr(R, name).restype = ctypes.POINTER(ctypes.c_uint)
         R.ref_scramble64_constants.restype = ctypes.POINTER(ctypes.c_ulonglong)
+        R.ref_mt19937_jump.restype = ctypes.POINTER(ctypes.c_uint)
+        R.ref_mt19937_set_limit.argtypes = [ctypes.c_void_p, ctypes.c_uint]
         # the reference tree has no 64-bit direction-vector tables: hand it the oracle's
         t = sobol_tables()
         _ref_keep.extend([t[2], t[3]])

@@ -30,9 +30,11 @@
 #include "rng/threefry.hpp"
 #include "rng/xorwow.hpp"
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 #include <memory>
+#include <thread>
 #include <vector>
 
 using namespace rocrand_impl::host;
@@ -64,6 +66,22 @@ void launch(unsigned blocks, unsigned threads, Kernel k, Args... args)
     for(unsigned b = 0; b < blocks; b++)
         for(unsigned t = 0; t < threads; t++)
             k(dim3(b), dim3(t), dim3(blocks), dim3(threads), args...);
+}
+
+// Blocks split over host threads: for kernels whose blocks write disjoint data.
+template<class Kernel, class... Args>
+void launch_parallel(unsigned blocks, unsigned threads, Kernel k, Args... args)
+{
+    const unsigned           T = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for(unsigned w = 0; w < T; w++)
+        pool.emplace_back([=] {
+            for(unsigned b = w; b < blocks; b += T)
+                for(unsigned t = 0; t < threads; t++)
+                    k(dim3(b), dim3(t), dim3(blocks), dim3(threads), args...);
+        });
+    for(auto& th : pool)
+        th.join();
 }
 
 // Scratch buffer whose address has the requested residue modulo 64 bytes, so the
@@ -405,8 +423,9 @@ struct mtgp_gen : gen_base
 // mt19937.hpp:793-908 init (jump_ahead_mt19937, init_engines_mt19937) and :911-1042
 // generate (short/long kernel choice and the start-input bookkeeping), over the default
 // grid 256 x 256 (config_defaults.hpp:76-80: 8192 generators of 8 threads).  The jump
-// kernel's host form costs ~10 ms per generator, so only the first `limit` generators
-// are set up (the others stay zero): callers keep every touched thread below 8*limit.
+// kernel's host form costs ~10 ms per generator (spread over host threads here); only the
+// first `limit` generators are set up (the others stay zero): callers keep every touched
+// thread below 8*limit.
 // The host form of the kernels shares one head/tail flag among eight emulated lanes
 // (mt19937.hpp:355-431), so only aligned calls of whole vectors are comparable.
 struct mt_gen : gen_base
@@ -420,7 +439,7 @@ struct mt_gen : gen_base
     void init()
     {
         std::vector<unsigned> base(size_t(G) * NN, 0u);
-        launch(limit, 128, jump_ahead_mt19937<128, CP, false>, base.data(), seed, rocrand_h_mt19937_jump);
+        launch_parallel(limit, 128, jump_ahead_mt19937<128, CP, false>, base.data(), seed, rocrand_h_mt19937_jump);
         engines.assign(size_t(NN / 8) * STRIDE, 0u);
         launch(256, 256, init_engines_mt19937<CP, false>, engines.data(), (const unsigned*)base.data());
         start_input = 0;

@@ -3,6 +3,8 @@
  *                       dimensions 1..19999 (rocrand_amd/data/make_joe_kuo.py)
  *   mtgp32_11213.bin  : the 512 MTGP32 parameter sets for exponent 11213
  *                       (rocrand_amd/data/make_mtgp32_params.py)
+ *   mt19937_jumps.bin : MT19937 jump polynomials x^(i*2^1000), x^(256*i*2^1000) mod phi
+ *                       (rocrand_amd/data/make_mt19937_jumps.c)
  * The build assembles this file with the data directory on the .incbin search path. */
     .section .note.GNU-stack,"",@progbits
     .section .rodata
@@ -22,3 +24,11 @@ rr_mtgp32_params_begin:
     .globl rr_mtgp32_params_end
     .hidden rr_mtgp32_params_end
 rr_mtgp32_params_end:
+    .p2align 8
+    .globl rr_mt19937_jumps_begin
+    .hidden rr_mt19937_jumps_begin
+rr_mt19937_jumps_begin:
+    .incbin "mt19937_jumps.bin"
+    .globl rr_mt19937_jumps_end
+    .hidden rr_mt19937_jumps_end
+rr_mt19937_jumps_end:

@@ -124,3 +124,42 @@ def test_discrete_distribution_argument_errors():
     assert L.rocrand_create_poisson_distribution(0.0, ctypes.byref(d)) == OUT_OF_RANGE
     assert L.rocrand_create_discrete_distribution(None, 0, 0, ctypes.byref(d)) == OUT_OF_RANGE
     assert L.rocrand_destroy_discrete_distribution(None) == OUT_OF_RANGE
+
+
+def test_mt_generators_host_decided_status_codes():
+    L = R.load()
+    for t in (R.MTGP32, R.MT19937):
+        st, g = _create(t)
+        assert st == SUCCESS
+        assert L.rocrand_set_seed(g, 77) == SUCCESS
+        assert L.rocrand_set_offset(g, 5) == TYPE_ERROR  # mtgp32.hpp:398-403, mt19937.hpp:757-762
+        assert L.rocrand_generate_long_long(g, None, 10) == TYPE_ERROR
+        assert L.rocrand_set_ordering(g, R.ORDERING_QUASI_DEFAULT) == OUT_OF_RANGE
+        assert L.rocrand_set_ordering(g, R.ORDERING_PSEUDO_LEGACY) == SUCCESS
+        # MT19937 accepts DEFAULT, BEST and LEGACY only (mt19937.hpp:765-779)
+        want = OUT_OF_RANGE if t == R.MT19937 else SUCCESS
+        assert L.rocrand_set_ordering(g, R.ORDERING_PSEUDO_DYNAMIC) == want
+        assert L.rocrand_destroy_generator(g) == SUCCESS
+
+
+@pytest.mark.parametrize("t", [R.SOBOL32, R.SCRAMBLED_SOBOL32, R.SOBOL64, R.SCRAMBLED_SOBOL64, R.QUASI_DEFAULT])
+def test_sobol_host_decided_status_codes(t):
+    L = R.load()
+    st, g = _create(t)
+    assert st == SUCCESS
+    try:
+        assert L.rocrand_set_seed(g, 5) == SUCCESS  # ignored (sobol.hpp:581-584)
+        assert L.rocrand_set_seed_uint4(g, R._uint4(1, 2, 3, 4)) == TYPE_ERROR
+        assert L.rocrand_set_offset(g, 1000) == SUCCESS
+        assert L.rocrand_set_ordering(g, R.ORDERING_PSEUDO_DEFAULT) == OUT_OF_RANGE
+        assert L.rocrand_set_ordering(g, R.ORDERING_QUASI_DEFAULT) == SUCCESS
+        assert L.rocrand_set_quasi_random_generator_dimensions(g, 0) == OUT_OF_RANGE
+        assert L.rocrand_set_quasi_random_generator_dimensions(g, 20001) == OUT_OF_RANGE
+        assert L.rocrand_set_quasi_random_generator_dimensions(g, 20000) == SUCCESS
+        # n % dimensions (sobol.hpp:628-631), decided before any launch
+        assert L.rocrand_set_quasi_random_generator_dimensions(g, 3) == SUCCESS
+        assert L.rocrand_generate(g, None, 10) == 105
+        if t in (R.SOBOL32, R.SCRAMBLED_SOBOL32, R.QUASI_DEFAULT):
+            assert L.rocrand_generate_long_long(g, None, 9) == TYPE_ERROR
+    finally:
+        assert L.rocrand_destroy_generator(g) == SUCCESS

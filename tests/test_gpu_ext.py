@@ -182,3 +182,76 @@ def test_mtgp32_set_offset_is_type_error():
     with pytest.raises(R.RocRandError) as e:
         g.offset = 5
     assert e.value.value == 103
+
+
+# ------------------------------------------------------------------ MT19937
+MT_CONFIGS = [(12345, 1000, 0), (0, 65536 + 777, 1), (0xFEDCBA9876543210, 13, 3), (7, 65536 * 80 + 5, 2)]
+
+
+@pytest.mark.parametrize("kind", list(_mtgp_cases()))
+def test_mt19937_parity(kind):
+    dt, meth, a = KINDS[kind]
+    a0 = a[0] if len(a) > 0 else 0.0
+    a1 = a[1] if len(a) > 1 else 0.0
+    for seed, n, shift in MT_CONFIGS:
+        if n > 100000 and kind not in ("u32", "uniform_f32", "normal_f32", "uniform_f64", "normal_f64", "u8"):
+            continue
+        print(f"case mt19937 {kind} seed={seed:#x} n={n} shift={shift}", flush=True)
+        g = R.PRNG(R.MT19937, seed=seed)
+        og = O.OracleGen(O.MT19937, seed=seed)
+        for _ in range(2):
+            buf = torch.zeros(n + 16, dtype=dt, device="cuda")
+            view = buf[shift:shift + n]
+            getattr(g, meth)(view, *a)
+            torch.cuda.synchronize()
+            st, want = og.generate(kind, n, view.data_ptr(), a0, a1)
+            assert st == 0
+            _check(kind, view.cpu().numpy(), want, (a0, a1))
+
+
+def test_mt19937_mixed_widths_and_rounds():
+    """One generator through changing input widths (1, 2, 4 words per input) and round
+    boundaries, misaligned; every call bit-exact (or within the normal bar)."""
+    g = R.PRNG(R.MT19937, seed=99)
+    og = O.OracleGen(O.MT19937, seed=99)
+    seq = [("u32", 65536 * 77 + 3, 1), ("uniform_f64", 1001, 1), ("normal_f64", 65536 * 20 + 7, 1),
+           ("u8", 4099, 3), ("uniform_f32", 65536 * 78, 0), ("normal_f32", 5, 1), ("u16", 65536 * 39 * 2 + 1, 1)]
+    for kind, n, shift in seq:
+        dt, meth, a = KINDS[kind]
+        print(f"case mt19937 sequence {kind} n={n} shift={shift}", flush=True)
+        buf = torch.zeros(n + 16, dtype=dt, device="cuda")
+        view = buf[shift:shift + n]
+        getattr(g, meth)(view, *a)
+        torch.cuda.synchronize()
+        a0 = a[0] if len(a) > 0 else 0.0
+        a1 = a[1] if len(a) > 1 else 0.0
+        st, want = og.generate(kind, n, view.data_ptr(), a0, a1)
+        assert st == 0
+        _check(kind, view.cpu().numpy(), want, (a0, a1))
+
+
+def test_mt19937_large():
+    """2^26 u32 then 2^26 uniform floats on one MT19937 generator, bit-exact."""
+    n = 1 << 26
+    g = R.PRNG(R.MT19937, seed=12345)
+    a = torch.empty(n, dtype=torch.uint32, device="cuda")
+    b = torch.empty(n, dtype=torch.float32, device="cuda")
+    g.generate(a)
+    g.uniform(b)
+    torch.cuda.synchronize()
+    og = O.OracleGen(O.MT19937, seed=12345)
+    st, wa = og.generate("u32", n, a.data_ptr())
+    st2, wb = og.generate("uniform_f32", n, b.data_ptr())
+    assert st == st2 == 0
+    assert np.array_equal(a.cpu().numpy(), wa)
+    assert np.array_equal(b.cpu().numpy(), wb)
+
+
+def test_mt19937_host_decided_errors():
+    g = R.PRNG(R.MT19937)
+    with pytest.raises(R.RocRandError) as e:
+        g.offset = 5
+    assert e.value.value == 103
+    with pytest.raises(R.RocRandError) as e:
+        g.set_ordering(R.ORDERING_PSEUDO_DYNAMIC)
+    assert e.value.value == 104

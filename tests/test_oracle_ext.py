@@ -193,3 +193,125 @@ def test_library_sobol_tables_equal_oracle(tables):
     assert st == 0 and np.array_equal(got, c32)
     st, got = get(L.rocrand_get_scramble_constants64, ctypes.c_ulonglong, 20000)
     assert st == 0 and np.array_equal(got, c64)
+
+
+# ------------------------------------------------------------------ MT19937
+MT_DATA = os.path.join(os.path.dirname(HERE), "rocrand_amd", "data")
+
+
+@pytest.fixture(scope="module")
+def jump_tool(tmp_path_factory):
+    """The library's jump-polynomial tool (rocrand_amd/data/make_mt19937_jumps.c), built here."""
+    import subprocess
+
+    exe = str(tmp_path_factory.mktemp("mtjump") / "make_mt19937_jumps")
+    subprocess.run(["gcc", "-O2", "-o", exe, os.path.join(MT_DATA, "make_mt19937_jumps.c")], check=True)
+    return exe
+
+
+def test_mt19937_jump_polynomial_matches_reference_vector(jump_tool):
+    """x^552552 mod phi equals the reference test's known polynomial (Berlekamp-Massey on
+    the recurrence, then square-and-multiply: a wrong phi or reduction fails)."""
+    import subprocess
+
+    gold = json.load(open(os.path.join(HERE, "golden", "mt19937_jump_552552.json")))
+    out = subprocess.run([jump_tool, "--power", "552552"], capture_output=True, text=True, check=True).stdout
+    assert [int(x) for x in out.split()] == gold["words"]
+
+
+def test_mt19937_jump_polynomial_brute_force(jump_tool):
+    """Horner with x^k mod phi equals k single steps of the word recurrence (k = 1000, 100003)."""
+    import subprocess
+
+    def comp(a, b, m):
+        y = (a & 0x80000000) | (b & 0x7FFFFFFF)
+        return m ^ (y >> 1) ^ (0x9908B0DF if y & 1 else 0)
+
+    s0 = [5489]
+    for i in range(1, 624):
+        s0.append((1812433253 * (s0[-1] ^ (s0[-1] >> 30)) + i) & 0xFFFFFFFF)
+    for k in (1000, 100003):
+        p = [int(x) for x in subprocess.run([jump_tool, "--power", str(k)], capture_output=True, text=True,
+                                            check=True).stdout.split()]
+        # Horner over the window (the reference jump kernel's evaluation), in numpy
+        seq = np.array(s0 + [0] * (19937 + k), dtype=np.uint64)
+        for i in range(624, 624 + max(k, 19937) + 1):
+            seq[i] = comp(int(seq[i - 624]), int(seq[i - 623]), int(seq[i - 227]))
+        bits = np.array([(p[j // 32] >> (j % 32)) & 1 for j in range(19937)], dtype=bool)
+        idx = np.nonzero(bits)[0]
+        jumped = np.array([np.bitwise_xor.reduce(seq[i + idx]) for i in range(624)], dtype=np.uint64)
+        want = seq[k:k + 624]
+        # the oldest word's low 31 bits never influence the sequence
+        assert jumped[0] >> 31 == want[0] >> 31
+        assert np.array_equal(jumped[1:], want[1:])
+
+
+def test_mt19937_jump_table_matches_reference():
+    if O.ref() is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    ref_jump = np.ctypeslib.as_array(O.ref().ref_mt19937_jump(), (510 * 624,)).copy()
+    mine = np.fromfile(os.path.join(MT_DATA, "mt19937_jumps.bin"), dtype="<u4")
+    assert np.array_equal(ref_jump, mine)
+
+
+@pytest.mark.parametrize("seed", [12345, 0xFEDCBA9876543210])
+def test_mt19937_generator0_is_std_mt19937(seed):
+    """Generator 0 (lanes 0-7 of thread block 0) is MT19937 seeded by init_genrand((seed >> 32) ^ seed):
+    numpy's RandomState implements the same recurrence, seeding and tempering."""
+    og = O.OracleGen(O.MT19937, seed=seed)
+    st, x = og.generate("u32", 65536 * 78, 0x1000)
+    assert st == 0
+    x = x.reshape(78, 65536)
+    special = [0, 113, 170, 283, 340, 397, 510, 567]
+    offs = [1, 57, 114, 171, 227, 284, 341, 398, 454, 511, 568]
+    g0 = np.zeros(624, dtype=np.uint32)
+    for lane in range(8):
+        for j in range(78):
+            e = special[lane] if j == 0 else offs[(j - 1) // 7] + 7 * lane + (j - 1) % 7
+            g0[e] = x[j, lane]
+    seedu = (seed ^ (seed >> 32)) & 0xFFFFFFFF
+    want = np.random.RandomState(seedu).randint(0, 2 ** 32, size=624, dtype=np.uint64).astype(np.uint32)
+    assert np.array_equal(g0, want)
+
+
+def test_mt19937_offset_is_type_error():
+    assert O.OracleGen(O.MT19937).set_offset(3) == 103
+
+
+@pytest.mark.parametrize("seed,calls", [
+    (12345, [("u32", 100), ("uniform_f32", 200), ("normal_f32", 50), ("uniform_f64", 20), ("u8", 40),
+             ("normal_f64", 10), ("u32", 64), ("poisson", 33)]),
+    (0xFEDCBA9876543210, [("u16", 256), ("lognormal_f32", 100), ("uniform_f16", 70), ("lognormal_f64", 6)]),
+])
+def test_mt19937_oracle_equals_reference_kernel_small(seed, calls):
+    """Aligned calls of whole vectors inside the first 64 generators (the host form of the
+    reference kernels shares one head/tail flag among eight emulated lanes)."""
+    R = O.ref()
+    if R is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    og = O.OracleGen(O.MT19937, seed=seed)
+    rg = O.RefGen(O.MT19937, seed=seed)
+    R.ref_mt19937_set_limit(rg.h, 64)
+    for kind, n in calls:
+        a, b = (10.0, 0.0) if kind == "poisson" else ((0.5, 1.5) if "normal" in kind else (0.0, 1.0))
+        s1, x = og.generate(kind, n, 0x10000, a, b)
+        s2, y = rg.generate(kind, n, 0x10000, a, b)
+        assert s1 == s2 == 0
+        assert _same(kind, x, y), (kind, n)
+
+
+def test_mt19937_oracle_equals_reference_kernel_rounds():
+    """Calls spanning several rounds of all 8192 generators, with input-width changes."""
+    R = O.ref()
+    if R is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    og = O.OracleGen(O.MT19937, seed=777)
+    rg = O.RefGen(O.MT19937, seed=777)
+    R.ref_mt19937_set_limit(rg.h, 8192)
+    for kind, n in [("u32", 65536 * 78 + 65536 * 3 + 100), ("uniform_f64", 65536 * 39 * 2 + 4),
+                    ("normal_f32", 1000000), ("u32", 65536 * 80)]:
+        a, b = (0.5, 1.5) if "normal" in kind else (0.0, 1.0)
+        s1, x = og.generate(kind, n, 0x10000, a, b)
+        s2, y = rg.generate(kind, n, 0x10000, a, b)
+        assert s1 == s2 == 0
+        assert _same(kind, x, y), (kind, n)
