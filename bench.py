@@ -98,6 +98,100 @@ def copy_calibration(torch, n):
     torch.cuda.synchronize()
 
 
+def batched_bench(R, torch, count=4096, seed=7, launches=10):
+    """4096 independent Philox4x32-10 / MRG32k3a streams (rocrand_amd.workloads.batch_mix):
+    sequential plain C-ABI call sequences (create, set_seed, set_offset, generate, destroy:
+    per-call set-up included) against one replay of the batch plan's HIP graph."""
+    import ctypes
+
+    from rocrand_amd.workloads import batch_mix
+
+    L = R.load()
+    specs = batch_mix(count, seed)
+    total_bytes = sum(sp.n * sp.itemsize for sp in specs)
+    offs, pos = [], 0
+    for sp in specs:
+        offs.append(pos)
+        pos += (sp.n * sp.itemsize + 255) // 256 * 256
+    buf = torch.empty(pos, dtype=torch.uint8, device="cuda")
+    base = buf.data_ptr()
+    streams = [R.BatchStream(sp.rng_type, sp.kind, sp.seed, sp.offset, sp.n, sp.a, sp.b, base + o)
+               for sp, o in zip(specs, offs)]
+    fn = {R.BATCH_UNIFORM_FLOAT: L.rocrand_generate_uniform, R.BATCH_NORMAL_FLOAT: L.rocrand_generate_normal,
+          R.BATCH_POISSON: L.rocrand_generate_poisson}
+
+    def plain_all():
+        for sp, o in zip(specs, offs):
+            g = ctypes.c_void_p()
+            R.check_rocrand(L.rocrand_create_generator(ctypes.byref(g), sp.rng_type))
+            L.rocrand_set_seed(g, sp.seed)
+            L.rocrand_set_offset(g, sp.offset)
+            if sp.kind == R.BATCH_UNIFORM_FLOAT:
+                st = fn[sp.kind](g, ctypes.c_void_p(base + o), sp.n)
+            elif sp.kind == R.BATCH_NORMAL_FLOAT:
+                st = fn[sp.kind](g, ctypes.c_void_p(base + o), sp.n, sp.a, sp.b)
+            else:
+                st = fn[sp.kind](g, ctypes.c_void_p(base + o), sp.n, sp.a)
+            R.check_rocrand(st)
+            L.rocrand_destroy_generator(g)
+        torch.cuda.synchronize()
+
+    plain_all()  # warm caches (Poisson table code, jump tables, kernels)
+    t = time.perf_counter()
+    plain_all()
+    plain_ms = (time.perf_counter() - t) * 1e3
+    ref = buf.clone()
+
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    plan = R.BatchPlan(streams, stream)
+    torch.cuda.synchronize()
+    plan_ms = (time.perf_counter() - t) * 1e3
+    buf.zero_()
+    torch.cuda.synchronize()
+    plan.launch()
+    torch.cuda.synchronize()
+    same = bool(torch.equal(buf, ref))
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(launches):
+        plan.launch()
+    end.record(stream)
+    torch.cuda.synchronize()
+    graph_ms = start.elapsed_time(end) / launches
+    plan.close()
+    del buf, ref
+    return {"streams": count, "workload_seed": seed, "values": sum(sp.n for sp in specs),
+            "bytes": total_bytes, "plain_ms": round(plain_ms, 2), "graph_launch_ms": round(graph_ms, 3),
+            "plan_create_ms": round(plan_ms, 1), "speedup": round(plain_ms / graph_ms, 1),
+            "graph_GBps": round(_gbs(total_bytes, graph_ms), 1), "outputs_equal_plain": same}
+
+
+def serving_bench(torch, rates=(1000, 10000, 50000), seconds=1.0, budget_gb=32):
+    """Open-loop Poisson arrivals at each rate for `seconds` (rocrand_amd.workloads.serving_mix:
+    sizes log-uniform in [256, 2^26], 50% Philox / 30% MRG32k3a / 20% Sobol64 x16 dims,
+    f32/f64/u32) through the continuous-batching server with a 32 GB HBM budget."""
+    from rocrand_amd.serving import Server
+    from rocrand_amd.workloads import serving_mix
+
+    srv = Server(torch, int(budget_gb * 1e9))
+    # warm-up: kernels, tables
+    srv.serve(serving_mix(50, 1e6, seed=1))
+    out = []
+    for rate in rates:
+        reqs = serving_mix(int(rate * seconds), rate, seed=11)
+        res = srv.serve(reqs)
+        out.append({"rate_rps": rate, "requests": len(reqs), "p50_ms": round(res.percentile(50) * 1e3, 3),
+                    "p99_ms": round(res.percentile(99) * 1e3, 3), "GBps": round(res.bytes / res.wall_s / 1e9, 1),
+                    "iterations": res.iterations, "max_batch": res.max_batch, "rejected": res.rejected,
+                    "wall_s": round(res.wall_s, 3)})
+        print(f"# serving: {out[-1]}", file=sys.stderr, flush=True)
+    del srv
+    return {"budget_GB": budget_gb, "workload_seed": 11, "runs": out}
+
+
 def load_traffic():
     """HBM bytes per launch of the headline kernel, from the committed counter summary."""
     path = os.path.join(HERE, "profiles", "traffic_philox_uniform_f32_2e28.json")
@@ -138,6 +232,8 @@ def main():
     ap.add_argument("--no-table", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=1 << 28)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain 2^28-float device copy (counter calibration)")
     args = ap.parse_args()
@@ -180,6 +276,10 @@ def main():
     ceiling = store_ceiling(torch, stream, n, 10)
     table = None if args.no_table else per_generator_table(R, torch, stream, n, 5)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
+    batched = None if args.no_batched else batched_bench(R, torch)
+    if batched:
+        print(f"# batched: {batched}", file=sys.stderr, flush=True)
+    serving = None if args.no_serving else serving_bench(torch)
 
     line = {
         "metric": "output GB/s & % of HBM write peak per generator; batched-vs-plain speedup",
@@ -202,6 +302,8 @@ def main():
                      "measured_store_ceiling_gbs": round(ceiling, 1)},
         "cpu_baseline": cpu,
         "per_generator": table,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 

@@ -416,19 +416,13 @@ protected:
         return p;
     }
 
-    rocrand_status generate_device(out_kind k, void* data, size_t n, const gen_args& args) override
+public:
+    ctr_kind kind() const { return m_kind; }
+
+    // Resolves the next generate call (head/tail split, word positions) without
+    // launching; c.params is left to the caller.
+    void prepare(const dist_shape& s, void* data, size_t n, counter_call& c)
     {
-        dist_shape     s;
-        rocrand_status st = shape(k, s);
-        if(st != ROCRAND_STATUS_SUCCESS)
-            return st;
-        if((st = init()) != ROCRAND_STATUS_SUCCESS)
-            return st;
-        if(data == nullptr)
-            return ROCRAND_STATUS_SUCCESS; // philox4x32_10.hpp:323-326
-        counter_call c{};
-        if((st = make_params(k, args, c.params)) != ROCRAND_STATUS_SUCCESS)
-            return st;
         const unsigned opt  = (unsigned)m_dim / s.in_w;
         const unsigned full = opt * s.out_w;
         const uintptr_t addr = reinterpret_cast<uintptr_t>(data);
@@ -447,6 +441,23 @@ protected:
         c.quirk     = (H != C0) ? 1 : 0;
         c.data      = data;
         c.n         = n;
+    }
+
+protected:
+    rocrand_status generate_device(out_kind k, void* data, size_t n, const gen_args& args) override
+    {
+        dist_shape     s;
+        rocrand_status st = shape(k, s);
+        if(st != ROCRAND_STATUS_SUCCESS)
+            return st;
+        if((st = init()) != ROCRAND_STATUS_SUCCESS)
+            return st;
+        if(data == nullptr)
+            return ROCRAND_STATUS_SUCCESS; // philox4x32_10.hpp:323-326
+        counter_call c{};
+        if((st = make_params(k, args, c.params)) != ROCRAND_STATUS_SUCCESS)
+            return st;
+        prepare(s, data, n, c);
         if(n > 0)
         {
             if(launch_counter(m_kind, k, false, c, stream) != hipSuccess)
@@ -580,8 +591,7 @@ public:
         return family_shape<word_family::w32>(k, false, s);
     }
 
-protected:
-    void reset() override { m_initialized = false; }
+    inter_kind kind() const { return m_kind; }
 
     // Seeded state of one engine before any jump, in the kernel's word order.
     void seed_state(uint32_t* w) const
@@ -629,6 +639,9 @@ protected:
         }
     }
 
+protected:
+    void reset() override { m_initialized = false; }
+
     rocrand_status generate_device(out_kind k, void* data, size_t n, const gen_args& args) override
     {
         dist_shape     s;
@@ -673,6 +686,35 @@ namespace rr
 {
 rocrand_generator make_counter_generator(rocrand_rng_type t) { return new counter_generator(t); }
 rocrand_generator make_interleaved_generator(rocrand_rng_type t) { return new interleaved_generator(t); }
+
+rocrand_status prepare_counter_call(rocrand_rng_type t, unsigned long long seed, unsigned long long offset,
+                                    out_kind k, void* data, size_t n, counter_call& c, dist_shape& s,
+                                    ctr_kind& kind)
+{
+    counter_generator g(t);
+    g.set_seed(seed);
+    g.set_offset(offset);
+    rocrand_status st = g.shape(k, s);
+    if(st != ROCRAND_STATUS_SUCCESS)
+        return st;
+    g.init();
+    g.prepare(s, data, n, c);
+    kind = g.kind();
+    return ROCRAND_STATUS_SUCCESS;
+}
+
+rocrand_status interleaved_seed_state(rocrand_rng_type t, unsigned long long seed, out_kind k, uint32_t w[6],
+                                      inter_kind& kind, dist_shape& s)
+{
+    interleaved_generator g(t);
+    g.set_seed(seed);
+    rocrand_status st = g.shape(k, s);
+    if(st != ROCRAND_STATUS_SUCCESS)
+        return st;
+    g.seed_state(w);
+    kind = g.kind();
+    return ROCRAND_STATUS_SUCCESS;
+}
 } // namespace rr
 
 // ============================================================================

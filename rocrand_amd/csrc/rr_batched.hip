@@ -1,0 +1,601 @@
+// rr_batched.hip -- many independent Philox4x32-10 / MRG32k3a generate calls in one launch.
+//
+// Each stream is the output of one fresh generator's single generate call (see
+// include/rocrand_batched.h).  Streams are grouped by (engine, output kind); a group is
+// one persistent kernel whose blocks walk a list of tiles:
+//   * Philox: a tile is up to 1024 consecutive vecs of one stream, computed exactly as
+//     the per-call kernels do (rr_counter_dev.hpp), so outputs match the plain path bit
+//     for bit; the stream's head and tail go to its first tile;
+//   * MRG32k3a: a tile is up to 256 of a stream's engines.  A lane initialises its engine
+//     from the stream's seeded state (jump by the engine index in subsequences and by
+//     offset / S + [engine < offset mod S] steps, as the per-call init kernel does) and
+//     then writes that engine's vecs v = id, id + S, ... (rr_interleaved_dev.hpp).  Only
+//     the engines a stream uses are set up: min(S, vec_n + 1) instead of S per call.
+// The plan records the group launches in a HIP graph once; a launch replays it.  The
+// grid size of each group is chosen by timing a few candidates on the first plan of a
+// given shape (kept in a process-wide cache).
+#include "rr_counter_dev.hpp"
+#include "rr_generator.hpp"
+#include "rr_interleaved_dev.hpp"
+
+#include <rocrand_batched.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+namespace rr
+{
+namespace
+{
+
+constexpr uint32_t CTR_TILE_VECS = 1024; // 4 wavefronts x 4 chunks of 64
+constexpr uint32_t MRG_TILE      = 256;
+
+struct batch_tile
+{
+    uint32_t stream;
+    uint32_t count; // vecs (counter) or engines (interleaved)
+    uint64_t first;
+};
+
+// One MRG32k3a stream: a fresh generator's single call.
+struct inter_call
+{
+    uint32_t    seeded[6];
+    uint32_t    start;    // offset mod S
+    uint64_t    base_off; // offset / S
+    void*       data;
+    size_t      n, mis, vec_n;
+    uint32_t    head, tail;
+    dist_params params;
+};
+
+template<ctr_kind K, class D>
+__global__ __launch_bounds__(256) void batch_counter_kernel(const counter_call* calls,
+                                                            const batch_tile*   tiles,
+                                                            uint32_t            ntiles)
+{
+    using namespace ctr_dev;
+    for(uint32_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x)
+    {
+        const batch_tile   tl  = tiles[ti];
+        const counter_call c   = calls[tl.stream];
+        const size_t       end = tl.first + tl.count;
+        if(fast_ok<K>(c))
+        {
+            const uint32_t T    = c.vec_base.sub ? 63u : 64u;
+            const size_t   wave = threadIdx.x >> 6;
+            for(size_t f = tl.first + wave * T; f < end; f += 4 * T)
+            {
+                if(c.vec_base.sub)
+                    fast_tile<K, D, true>(c, f, end);
+                else
+                    fast_tile<K, D, false>(c, f, end);
+            }
+        }
+        else
+        {
+            for(size_t v = tl.first + threadIdx.x; v < end; v += 256)
+                generic_vec<K, D>(c, v);
+        }
+        if(tl.first == 0 && threadIdx.x == 0)
+            head_tail<K, D>(c);
+    }
+}
+
+template<inter_kind K, class D>
+__global__ __launch_bounds__(256) void batch_inter_kernel(const inter_call* calls,
+                                                          const batch_tile* tiles,
+                                                          uint32_t          ntiles,
+                                                          const device_tables t)
+{
+    using namespace inter_dev;
+    using out_t         = typename D::out_t;
+    constexpr int IN    = D::in_w;
+    constexpr int OUT   = D::out_w;
+    constexpr int BYTES = OUT * (int)sizeof(out_t);
+    for(uint32_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x)
+    {
+        const batch_tile tl = tiles[ti];
+        const uint32_t   id = (uint32_t)tl.first + threadIdx.x;
+        if(threadIdx.x >= tl.count)
+            continue;
+        const inter_call& c   = calls[tl.stream];
+        const uint32_t    eid = (id + c.start) % S;
+        ieng<K>           g;
+        g.set(c.seeded);
+        g.init(t, eid, c.base_off + (eid < c.start ? 1u : 0u));
+        out_t* const data     = static_cast<out_t*>(c.data);
+        out_t* const vec_data = data + c.mis;
+        alignas(16) out_t o[OUT];
+        uint32_t     in[IN];
+        size_t       index = id;
+        for(; index < c.vec_n; index += S)
+        {
+#pragma unroll
+            for(int i = 0; i < IN; i++)
+                in[i] = g.next();
+            D::apply(in, o, c.params);
+            if constexpr(BYTES == 4)
+                *reinterpret_cast<uint32_t*>(vec_data + index * OUT) = *reinterpret_cast<const uint32_t*>(o);
+            else if constexpr(BYTES == 8)
+                *reinterpret_cast<uint2*>(vec_data + index * OUT) = *reinterpret_cast<const uint2*>(o);
+            else
+                *reinterpret_cast<uint4*>(vec_data + index * OUT) = *reinterpret_cast<const uint4*>(o);
+        }
+        // the engine that would produce vec index vec_n writes the head, then the tail
+        // (mrg.hpp:114-153)
+        if(OUT > 1 && index == c.vec_n)
+        {
+            if(c.head > 0)
+            {
+#pragma unroll
+                for(int i = 0; i < IN; i++)
+                    in[i] = g.next();
+                D::apply(in, o, c.params);
+                for(uint32_t j = 0; j < (uint32_t)OUT; j++)
+                    if(j < c.head)
+                        data[j] = o[j];
+            }
+            if(c.tail > 0)
+            {
+#pragma unroll
+                for(int i = 0; i < IN; i++)
+                    in[i] = g.next();
+                D::apply(in, o, c.params);
+                for(uint32_t j = 0; j < (uint32_t)OUT; j++)
+                    if(j < c.tail)
+                        data[c.n - c.tail + j] = o[j];
+            }
+        }
+    }
+}
+
+out_kind kind_of(rocrand_batch_kind k)
+{
+    switch(k)
+    {
+        case ROCRAND_BATCH_UINT: return out_kind::u32;
+        case ROCRAND_BATCH_UNIFORM_FLOAT: return out_kind::uniform_f32;
+        case ROCRAND_BATCH_UNIFORM_DOUBLE: return out_kind::uniform_f64;
+        case ROCRAND_BATCH_NORMAL_FLOAT: return out_kind::normal_f32;
+        case ROCRAND_BATCH_NORMAL_DOUBLE: return out_kind::normal_f64;
+        case ROCRAND_BATCH_LOG_NORMAL_FLOAT: return out_kind::lognormal_f32;
+        case ROCRAND_BATCH_LOG_NORMAL_DOUBLE: return out_kind::lognormal_f64;
+        default: return out_kind::poisson;
+    }
+}
+
+using launcher = hipError_t (*)(const void* calls, const batch_tile* tiles, uint32_t ntiles, uint32_t blocks,
+                                const device_tables& t, hipStream_t s);
+
+template<ctr_kind K, class D>
+hipError_t launch_counter_group(const void* calls, const batch_tile* tiles, uint32_t ntiles, uint32_t blocks,
+                                const device_tables&, hipStream_t s)
+{
+    hipLaunchKernelGGL((batch_counter_kernel<K, D>), dim3(blocks), dim3(256), 0, s,
+                       static_cast<const counter_call*>(calls), tiles, ntiles);
+    return hipGetLastError();
+}
+
+template<inter_kind K, class D>
+hipError_t launch_inter_group(const void* calls, const batch_tile* tiles, uint32_t ntiles, uint32_t blocks,
+                              const device_tables& t, hipStream_t s)
+{
+    hipLaunchKernelGGL((batch_inter_kernel<K, D>), dim3(blocks), dim3(256), 0, s,
+                       static_cast<const inter_call*>(calls), tiles, ntiles, t);
+    return hipGetLastError();
+}
+
+launcher pick_launcher(bool philox, out_kind k)
+{
+    constexpr word_family W = word_family::w32, M = word_family::mrg32;
+    constexpr ctr_kind    P = ctr_kind::philox4x32;
+    constexpr inter_kind  R = inter_kind::mrg32k3a;
+#define RR_PICK(KIND)                                                                                  \
+    case KIND:                                                                                         \
+        return philox ? &launch_counter_group<P, dist<W, KIND>> : &launch_inter_group<R, dist<M, KIND>>;
+    switch(k)
+    {
+        RR_PICK(out_kind::u32)
+        RR_PICK(out_kind::uniform_f32)
+        RR_PICK(out_kind::uniform_f64)
+        RR_PICK(out_kind::normal_f32)
+        RR_PICK(out_kind::normal_f64)
+        RR_PICK(out_kind::lognormal_f32)
+        RR_PICK(out_kind::lognormal_f64)
+        RR_PICK(out_kind::poisson)
+        default: return nullptr;
+    }
+#undef RR_PICK
+}
+
+struct group
+{
+    bool                    philox;
+    out_kind                kind;
+    launcher                fn;
+    std::vector<counter_call> ccalls;
+    std::vector<inter_call>   icalls;
+    std::vector<batch_tile>   tiles;
+    size_t                  bytes_out = 0;
+    void*                   d_calls   = nullptr;
+    batch_tile*             d_tiles   = nullptr;
+    uint32_t                blocks    = 2048;
+};
+
+// Grid sizes chosen per (engine, kind, log2 of the group's output bytes).
+std::mutex                        g_tune_mtx;
+std::map<std::tuple<int, int, int>, uint32_t> g_tuned;
+
+} // namespace
+} // namespace rr
+
+using namespace rr;
+
+struct rocrand_batch_plan_st
+{
+    hipStream_t                stream = 0;
+    std::vector<group>         groups;
+    std::vector<double*>       d_prob;
+    std::vector<unsigned int*> d_alias;
+    device_tables              tables{};
+    hipGraph_t                 graph = nullptr;
+    hipGraphExec_t             exec  = nullptr;
+
+    ~rocrand_batch_plan_st()
+    {
+        if(exec)
+            (void)hipGraphExecDestroy(exec);
+        if(graph)
+            (void)hipGraphDestroy(graph);
+        for(auto& g : groups)
+        {
+            (void)hipFree(g.d_calls);
+            (void)hipFree(g.d_tiles);
+        }
+        for(auto* p : d_prob)
+            (void)hipFree(p);
+        for(auto* p : d_alias)
+            (void)hipFree(p);
+    }
+
+    hipError_t run_groups(hipStream_t s) const
+    {
+        for(const auto& g : groups)
+        {
+            if(g.tiles.empty())
+                continue;
+            const uint32_t blocks = std::min<uint32_t>(g.blocks, (uint32_t)g.tiles.size());
+            hipError_t     e      = g.fn(g.d_calls, g.d_tiles, (uint32_t)g.tiles.size(), blocks, tables, s);
+            if(e != hipSuccess)
+                return e;
+        }
+        return hipSuccess;
+    }
+};
+
+namespace
+{
+
+rocrand_status build(rocrand_batch_plan_st& p, const rocrand_batch_stream* streams, size_t count)
+{
+    // Poisson alias tables, one per distinct lambda (poisson.hpp:110-161, discrete.hpp:189-265)
+    std::map<double, alias_table> tables;
+    std::map<std::pair<int, int>, size_t> gid; // (philox, kind) -> group index
+    for(size_t i = 0; i < count; i++)
+    {
+        const rocrand_batch_stream& b = streams[i];
+        const bool philox = b.rng_type == ROCRAND_RNG_PSEUDO_PHILOX4_32_10;
+        if(!philox && b.rng_type != ROCRAND_RNG_PSEUDO_MRG32K3A)
+            return ROCRAND_STATUS_TYPE_ERROR;
+        if((int)b.kind < 0 || (int)b.kind > 7)
+            return ROCRAND_STATUS_TYPE_ERROR;
+        if(b.n > 0 && b.output == nullptr)
+            return ROCRAND_STATUS_OUT_OF_RANGE;
+        const out_kind k = kind_of(b.kind);
+        dist_params    prm{};
+        prm.mf = (float)b.a;
+        prm.sf = (float)b.b;
+        prm.md = b.a;
+        prm.sd = b.b;
+        if(k == out_kind::poisson)
+        {
+            if(!(b.a > 0.0))
+                return ROCRAND_STATUS_OUT_OF_RANGE; // rocrand.cpp:371-374
+            prm.lambda      = b.a;
+            prm.sqrt_lambda = std::sqrt(b.a);
+            prm.huge        = b.a > 4000.0 ? 1 : 0;
+            if(!prm.huge)
+            {
+                auto it = tables.find(b.a);
+                if(it == tables.end())
+                {
+                    std::vector<double>       prob;
+                    std::vector<unsigned int> alias;
+                    unsigned int              off  = 0;
+                    const unsigned int        size = poisson_table(b.a, off, prob, alias);
+                    double*                   dp   = nullptr;
+                    unsigned int*             da   = nullptr;
+                    if(hipMalloc(&dp, size * sizeof(double)) != hipSuccess)
+                        return ROCRAND_STATUS_ALLOCATION_FAILED;
+                    p.d_prob.push_back(dp);
+                    if(hipMalloc(&da, size * sizeof(unsigned int)) != hipSuccess)
+                        return ROCRAND_STATUS_ALLOCATION_FAILED;
+                    p.d_alias.push_back(da);
+                    if(hipMemcpy(dp, prob.data(), size * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
+                       || hipMemcpy(da, alias.data(), size * sizeof(unsigned int), hipMemcpyHostToDevice)
+                              != hipSuccess)
+                        return ROCRAND_STATUS_INTERNAL_ERROR;
+                    alias_table t{};
+                    t.size   = size;
+                    t.offset = off;
+                    t.prob   = dp;
+                    t.alias  = da;
+                    it       = tables.emplace(b.a, t).first;
+                }
+                prm.pt = it->second;
+            }
+        }
+        auto key = std::make_pair(philox ? 1 : 0, (int)k);
+        auto git = gid.find(key);
+        if(git == gid.end())
+        {
+            group g;
+            g.philox = philox;
+            g.kind   = k;
+            g.fn     = pick_launcher(philox, k);
+            if(!g.fn)
+                return ROCRAND_STATUS_TYPE_ERROR;
+            p.groups.push_back(std::move(g));
+            git = gid.emplace(key, p.groups.size() - 1).first;
+        }
+        group&         g  = p.groups[git->second];
+        const uint32_t si = (uint32_t)(philox ? g.ccalls.size() : g.icalls.size());
+        dist_shape     s{};
+        if(philox)
+        {
+            counter_call   c{};
+            ctr_kind       ck;
+            rocrand_status st = prepare_counter_call(b.rng_type, b.seed, b.offset, k, b.output, b.n, c, s, ck);
+            if(st != ROCRAND_STATUS_SUCCESS)
+                return st;
+            c.params = prm;
+            g.ccalls.push_back(c);
+            // at least one tile per stream: its first tile also writes head and tail
+            size_t v = 0;
+            do
+            {
+                const uint32_t cnt = (uint32_t)std::min<size_t>(CTR_TILE_VECS, c.vec_n - v);
+                g.tiles.push_back(batch_tile{si, cnt, v});
+                v += cnt;
+            } while(v < c.vec_n);
+        }
+        else
+        {
+            inter_call     c{};
+            inter_kind     ik;
+            rocrand_status st = interleaved_seed_state(b.rng_type, b.seed, k, c.seeded, ik, s);
+            if(st != ROCRAND_STATUS_SUCCESS)
+                return st;
+            const uintptr_t addr = reinterpret_cast<uintptr_t>(b.output);
+            c.start    = (uint32_t)(b.offset % INTERLEAVED_ENGINES);
+            c.base_off = b.offset / INTERLEAVED_ENGINES;
+            c.data     = b.output;
+            c.n        = b.n;
+            c.mis      = (s.out_w - (addr / s.esz) % s.out_w) % s.out_w;
+            c.head     = (uint32_t)std::min<size_t>(b.n, c.mis);
+            c.tail     = (uint32_t)((b.n - c.head) % s.out_w);
+            c.vec_n    = (b.n - c.head) / s.out_w;
+            c.params   = prm;
+            g.icalls.push_back(c);
+            const bool   extra = s.out_w > 1 && (c.head > 0 || c.tail > 0);
+            const size_t lanes = std::min<size_t>(INTERLEAVED_ENGINES, c.vec_n + (extra ? 1 : 0));
+            for(size_t e = 0; e < lanes; e += MRG_TILE)
+                g.tiles.push_back(batch_tile{si, (uint32_t)std::min<size_t>(MRG_TILE, lanes - e), e});
+        }
+        g.bytes_out += b.n * (size_t)s.esz;
+    }
+    for(auto& g : p.groups)
+    {
+        const size_t cb = g.philox ? g.ccalls.size() * sizeof(counter_call) : g.icalls.size() * sizeof(inter_call);
+        const void*  src = g.philox ? (const void*)g.ccalls.data() : (const void*)g.icalls.data();
+        if(hipMalloc(&g.d_calls, std::max<size_t>(cb, 16)) != hipSuccess
+           || hipMalloc(&g.d_tiles, std::max<size_t>(g.tiles.size() * sizeof(batch_tile), 16)) != hipSuccess)
+            return ROCRAND_STATUS_ALLOCATION_FAILED;
+        if(hipMemcpy(g.d_calls, src, cb, hipMemcpyHostToDevice) != hipSuccess
+           || hipMemcpy(g.d_tiles, g.tiles.data(), g.tiles.size() * sizeof(batch_tile), hipMemcpyHostToDevice)
+                  != hipSuccess)
+            return ROCRAND_STATUS_INTERNAL_ERROR;
+    }
+    if(get_device_tables(p.tables) != hipSuccess)
+        return ROCRAND_STATUS_INTERNAL_ERROR;
+    return ROCRAND_STATUS_SUCCESS;
+}
+
+// Chooses each group's grid size: the cached choice for its shape, else the fastest of a
+// few candidates timed on the plan's own outputs (the values written are the same for
+// every grid size).
+rocrand_status tune(rocrand_batch_plan_st& p)
+{
+    static const uint32_t candidates[] = {1024, 2048, 4096, 8192, 16384};
+    for(auto& g : p.groups)
+    {
+        if(g.tiles.empty())
+            continue;
+        const int  lg  = g.bytes_out ? (int)std::log2((double)g.bytes_out) : 0;
+        const auto key = std::make_tuple(g.philox ? 1 : 0, (int)g.kind, lg);
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mtx);
+            auto                        it = g_tuned.find(key);
+            if(it != g_tuned.end())
+            {
+                g.blocks = it->second;
+                continue;
+            }
+        }
+        hipEvent_t a, b;
+        if(hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+            return ROCRAND_STATUS_INTERNAL_ERROR;
+        float    best_ms = 1e30f;
+        uint32_t best    = g.blocks;
+        for(uint32_t c : candidates)
+        {
+            const uint32_t blocks = std::min<uint32_t>(c, (uint32_t)g.tiles.size());
+            // one untimed run, then one timed
+            for(int rep = 0; rep < 2; rep++)
+            {
+                if(rep == 1 && hipEventRecord(a, p.stream) != hipSuccess)
+                    return ROCRAND_STATUS_INTERNAL_ERROR;
+                if(g.fn(g.d_calls, g.d_tiles, (uint32_t)g.tiles.size(), blocks, p.tables, p.stream) != hipSuccess)
+                    return ROCRAND_STATUS_LAUNCH_FAILURE;
+            }
+            float ms = 0.f;
+            if(hipEventRecord(b, p.stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess
+               || hipEventElapsedTime(&ms, a, b) != hipSuccess)
+                return ROCRAND_STATUS_INTERNAL_ERROR;
+            if(ms < best_ms)
+            {
+                best_ms = ms;
+                best    = c;
+            }
+            if(blocks == g.tiles.size())
+                break; // larger grids change nothing
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        g.blocks = best;
+        std::lock_guard<std::mutex> lock(g_tune_mtx);
+        g_tuned[key] = best;
+    }
+    return ROCRAND_STATUS_SUCCESS;
+}
+
+} // namespace
+
+extern "C" {
+
+rocrand_status ROCRANDAPI rocrand_batch_plan_create(rocrand_batch_plan*         plan,
+                                                    const rocrand_batch_stream* streams,
+                                                    size_t                      count,
+                                                    hipStream_t                 stream)
+{
+    if(plan == nullptr || (streams == nullptr && count > 0))
+        return ROCRAND_STATUS_OUT_OF_RANGE;
+    *plan = nullptr;
+    rocrand_batch_plan_st* p = nullptr;
+    try
+    {
+        p         = new rocrand_batch_plan_st();
+        p->stream = stream;
+        rocrand_status st = build(*p, streams, count);
+        if(st == ROCRAND_STATUS_SUCCESS)
+            st = tune(*p);
+        if(st == ROCRAND_STATUS_SUCCESS)
+        {
+            // Record the group launches once.  The null stream cannot be captured: record
+            // on a private stream (ordered after the plan's stream) and launch on the
+            // plan's stream.
+            hipStream_t cap = stream;
+            if(cap == 0 && hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess)
+                st = ROCRAND_STATUS_INTERNAL_ERROR;
+            if(st == ROCRAND_STATUS_SUCCESS && cap != stream && hipStreamSynchronize(stream) != hipSuccess)
+                st = ROCRAND_STATUS_INTERNAL_ERROR;
+            if(st == ROCRAND_STATUS_SUCCESS && hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed) != hipSuccess)
+                st = ROCRAND_STATUS_INTERNAL_ERROR;
+            else if(st == ROCRAND_STATUS_SUCCESS)
+            {
+                const hipError_t le = p->run_groups(cap);
+                const hipError_t ce = hipStreamEndCapture(cap, &p->graph);
+                if(le != hipSuccess || ce != hipSuccess
+                   || hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) != hipSuccess)
+                    st = ROCRAND_STATUS_INTERNAL_ERROR;
+            }
+            if(cap != stream && cap != 0)
+                (void)hipStreamDestroy(cap);
+        }
+        if(st != ROCRAND_STATUS_SUCCESS)
+        {
+            (void)hipStreamSynchronize(stream);
+            delete p;
+            return st;
+        }
+    }
+    catch(const std::bad_alloc&)
+    {
+        delete p;
+        return ROCRAND_STATUS_ALLOCATION_FAILED;
+    }
+    catch(rocrand_status st)
+    {
+        delete p;
+        return st;
+    }
+    *plan = p;
+    return ROCRAND_STATUS_SUCCESS;
+}
+
+rocrand_status ROCRANDAPI rocrand_batch_plan_launch(rocrand_batch_plan plan)
+{
+    if(plan == nullptr)
+        return ROCRAND_STATUS_NOT_CREATED;
+    if(plan->groups.empty())
+        return ROCRAND_STATUS_SUCCESS;
+    return hipGraphLaunch(plan->exec, plan->stream) == hipSuccess ? ROCRAND_STATUS_SUCCESS
+                                                                  : ROCRAND_STATUS_LAUNCH_FAILURE;
+}
+
+rocrand_status ROCRANDAPI rocrand_batch_plan_destroy(rocrand_batch_plan plan)
+{
+    if(plan == nullptr)
+        return ROCRAND_STATUS_NOT_CREATED;
+    (void)hipStreamSynchronize(plan->stream);
+    delete plan;
+    return ROCRAND_STATUS_SUCCESS;
+}
+
+rocrand_status ROCRANDAPI rocrand_generate_batched(const rocrand_batch_stream* streams, size_t count,
+                                                   hipStream_t stream)
+{
+    if(streams == nullptr && count > 0)
+        return ROCRAND_STATUS_OUT_OF_RANGE;
+    try
+    {
+        rocrand_batch_plan_st p;
+        p.stream          = stream;
+        rocrand_status st = build(p, streams, count);
+        if(st != ROCRAND_STATUS_SUCCESS)
+            return st;
+        for(auto& g : p.groups)
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mtx);
+            const int  lg = g.bytes_out ? (int)std::log2((double)g.bytes_out) : 0;
+            auto       it = g_tuned.find(std::make_tuple(g.philox ? 1 : 0, (int)g.kind, lg));
+            if(it != g_tuned.end())
+                g.blocks = it->second;
+        }
+        if(p.run_groups(stream) != hipSuccess)
+        {
+            (void)hipStreamSynchronize(stream);
+            return ROCRAND_STATUS_LAUNCH_FAILURE;
+        }
+        // the plan's device buffers are released when this returns: wait for the kernels
+        if(hipStreamSynchronize(stream) != hipSuccess)
+            return ROCRAND_STATUS_INTERNAL_ERROR;
+    }
+    catch(const std::bad_alloc&)
+    {
+        return ROCRAND_STATUS_ALLOCATION_FAILED;
+    }
+    catch(rocrand_status st)
+    {
+        return st;
+    }
+    return ROCRAND_STATUS_SUCCESS;
+}
+
+} // extern "C"

@@ -1,287 +1,38 @@
 // rr_counter.hip -- Philox4x32-10 and ThreeFry{2,4}x{32,64}-20 bulk generation on gfx950.
 //
-// Output order reproduces generate_philox (library/src/rng/philox4x32_10.hpp:106-214)
-// and generate_threefry (library/src/rng/threefry.hpp:86-198) under the DEFAULT
-// grid: vec index v (one engine block of DIM words, FULL outputs) starts at word
-// position base + DIM*v, where base = engine position + H, except that vec
-// indices that are multiples of the reference grid stride start at engine
-// position + C0 (the words the head actually consumed).  The reference's grid is
-// not this kernel's grid: any schedule that computes that map is equivalent.
+// The output map is described in rr_counter_dev.hpp.  The reference's grid is not this
+// kernel's grid: any schedule that computes that map is equivalent.
 //
 // Schedule: one vec per lane over a grid that covers the whole call, so the
 // resident wavefronts always write one contiguous front of the output (16-byte
 // stores, 1 KiB per wavefront).  A grid-stride loop over a small grid spreads the
 // concurrent stores over the whole buffer and was measured 1.5x slower on MI355X.
-#include "rr_launch.hpp"
+#include "rr_counter_dev.hpp"
 
 namespace rr
 {
 namespace
 {
 
-template<class W, int DIM>
-__device__ __forceinline__ void interleave(const W* a, const W* b, uint32_t sub, W* o)
-{
-    if constexpr(DIM == 4)
-    {
-        switch(sub)
-        {
-            case 0: o[0] = a[0]; o[1] = a[1]; o[2] = a[2]; o[3] = a[3]; break;
-            case 1: o[0] = a[1]; o[1] = a[2]; o[2] = a[3]; o[3] = b[0]; break;
-            case 2: o[0] = a[2]; o[1] = a[3]; o[2] = b[0]; o[3] = b[1]; break;
-            default: o[0] = a[3]; o[1] = b[0]; o[2] = b[1]; o[3] = b[2]; break;
-        }
-    }
-    else
-    {
-        if(sub == 0)
-        {
-            o[0] = a[0];
-            o[1] = a[1];
-        }
-        else
-        {
-            o[0] = a[1];
-            o[1] = b[0];
-        }
-    }
-}
-
-// Sequential reader over the word stream (the engine's next()), for head/tail.
-template<ctr_kind K>
-struct seq_reader
-{
-    using T    = ctr_traits<K>;
-    using W    = typename T::word;
-    static constexpr int DIM = T::DIM;
-    W        ctr[DIM];
-    W        cur[DIM];
-    uint32_t sub;
-    __device__ seq_reader(const ctr_pos& p, uint64_t add_ctr, const ctr_key& key)
-    {
-#pragma unroll
-        for(int i = 0; i < DIM; i++)
-            ctr[i] = (W)p.c[i];
-        ctr_add<W, DIM>(ctr, add_ctr);
-        sub = p.sub;
-        T::block(ctr, key, cur);
-    }
-    __device__ W next(const ctr_key& key)
-    {
-        W r = cur[0];
-#pragma unroll
-        for(int i = 1; i < DIM; i++)
-            if((int)sub == i)
-                r = cur[i];
-        sub++;
-        if((int)sub == DIM)
-        {
-            sub = 0;
-            ctr_add<W, DIM>(ctr, 1);
-            T::block(ctr, key, cur);
-        }
-        return r;
-    }
-};
-
-template<int BYTES>
-__device__ __forceinline__ void store_vec(void* dst, const uint32_t* u)
-{
-    if constexpr(BYTES == 8)
-    {
-        *reinterpret_cast<uint2*>(dst) = make_uint2(u[0], u[1]);
-    }
-    else
-    {
-#pragma unroll
-        for(int j = 0; j < BYTES / 16; j++)
-            reinterpret_cast<uint4*>(dst)[j] = make_uint4(u[4 * j], u[4 * j + 1], u[4 * j + 2], u[4 * j + 3]);
-    }
-}
-
-// Head and tail of a call (at most FULL-1 outputs each), written by one thread.
-template<ctr_kind K, class D>
-__device__ void head_tail(const counter_call& c)
-{
-    using T                    = ctr_traits<K>;
-    using W                    = typename T::word;
-    using out_t                = typename D::out_t;
-    constexpr int DIM          = T::DIM;
-    constexpr int IN           = D::in_w;
-    constexpr int OUT          = D::out_w;
-    constexpr int OPT          = DIM / IN;
-    out_t* const  data         = static_cast<out_t*>(c.data);
-    {
-        out_t tmp[OUT];
-        if(c.head > 0)
-        {
-            // Head: consecutive words from the engine position (thread 0 of the reference grid).
-            seq_reader<K> r(c.pos, 0, c.key);
-            for(int s = 0; s < OPT; s++)
-            {
-                if((uint32_t)(s * OUT) >= c.head)
-                    break;
-                W in[IN];
-#pragma unroll
-                for(int i = 0; i < IN; i++)
-                    in[i] = r.next(c.key);
-                D::apply(in, tmp, c.params);
-                for(int o = 0; o < OUT; o++)
-                    if((uint32_t)(s * OUT + o) < c.head)
-                        data[s * OUT + o] = tmp[o];
-            }
-        }
-        if(c.tail > 0)
-        {
-            // Tail: the thread whose next vec index is vec_n continues from that vec's position.
-            const ctr_pos& b = (c.quirk && (c.vec_n % CTR_REF_STRIDE) == 0) ? c.vec_base0 : c.vec_base;
-            seq_reader<K>  r(b, c.vec_n, c.key);
-            for(int s = 0; s < OPT; s++)
-            {
-                if((uint32_t)(s * OUT) >= c.tail)
-                    break;
-                W in[IN];
-#pragma unroll
-                for(int i = 0; i < IN; i++)
-                    in[i] = r.next(c.key);
-                D::apply(in, tmp, c.params);
-                for(int o = 0; o < OUT; o++)
-                    if((uint32_t)(s * OUT + o) < c.tail)
-                        data[c.n - c.tail + s * OUT + o] = tmp[o];
-            }
-        }
-    }
-
-}
+using namespace ctr_dev;
 
 template<ctr_kind K, class D>
 __global__ __launch_bounds__(256) void counter_kernel(counter_call c)
 {
-    using T                    = ctr_traits<K>;
-    using W                    = typename T::word;
-    using out_t                = typename D::out_t;
-    constexpr int DIM          = T::DIM;
-    constexpr int IN           = D::in_w;
-    constexpr int OUT          = D::out_w;
-    constexpr int OPT          = DIM / IN;
-    constexpr int FULL         = OPT * OUT;
-    constexpr int BYTES        = FULL * (int)sizeof(out_t);
-    static_assert(DIM % IN == 0, "input width");
-    static_assert(BYTES == 8 || BYTES == 16 || BYTES == 32, "vec size");
-
-    out_t* const  data     = static_cast<out_t*>(c.data);
-    out_t* const  vec_data = data + c.mis;
-    const size_t  stride   = (size_t)gridDim.x * blockDim.x;
-
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
     for(size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < c.vec_n; v += stride)
-    {
-        const ctr_pos& b = (c.quirk && (v % CTR_REF_STRIDE) == 0) ? c.vec_base0 : c.vec_base;
-        W              ctr[DIM];
-#pragma unroll
-        for(int i = 0; i < DIM; i++)
-            ctr[i] = (W)b.c[i];
-        ctr_add<W, DIM>(ctr, v);
-        W w[DIM];
-        T::block(ctr, c.key, w);
-        W words[DIM];
-        if(b.sub != 0)
-        {
-            W w2[DIM];
-            ctr_add<W, DIM>(ctr, 1);
-            T::block(ctr, c.key, w2);
-            interleave<W, DIM>(w, w2, b.sub, words);
-        }
-        else
-        {
-#pragma unroll
-            for(int i = 0; i < DIM; i++)
-                words[i] = w[i];
-        }
-        out_t o[FULL];
-#pragma unroll
-        for(int s = 0; s < OPT; s++)
-            D::apply(words + s * IN, o + s * OUT, c.params);
-        uint32_t u[BYTES / 4];
-        __builtin_memcpy(u, o, BYTES);
-        store_vec<BYTES>(vec_data + v * FULL, u);
-    }
-
+        generic_vec<K, D>(c, v);
     if(blockIdx.x == 0 && threadIdx.x == 0)
         head_tail<K, D>(c);
 }
 
-
-// Fast path for the 32-bit-word engines when every vec shares one engine position
-// modulo the block (no head-shift quirk) and the counter's words above bit 63 do
-// not change within the call: the base lives in scalar registers and each lane
-// runs exactly one engine block per vec.  When the stream starts mid-block
-// (SUB), a wavefront covers 63 vecs with 64 consecutive blocks and each lane
-// takes the words it is missing from its neighbour's block (one cross-lane
-// shuffle per word) instead of running a second block.
 template<ctr_kind K, class D, bool SUB>
 __global__ __launch_bounds__(256) void counter_fast_kernel(counter_call c)
 {
-    using T                = ctr_traits<K>;
-    using W                = typename T::word;
-    using out_t            = typename D::out_t;
-    constexpr int DIM      = T::DIM;
-    constexpr int IN       = D::in_w;
-    constexpr int OUT      = D::out_w;
-    constexpr int OPT      = DIM / IN;
-    constexpr int FULL     = OPT * OUT;
-    constexpr int BYTES    = FULL * (int)sizeof(out_t);
     constexpr uint32_t TILE = SUB ? 63u : 64u;
-    static_assert(sizeof(W) == 4, "32-bit-word engines only");
-
-    const uint64_t clo  = c.vec_base.c[0] | (c.vec_base.c[1] << 32);
-    const W        c2   = (W)c.vec_base.c[2];
-    const W        c3   = (W)c.vec_base.c[3];
-    const uint32_t sub  = c.vec_base.sub;
-    const uint32_t lane = threadIdx.x & 63u;
-    const size_t   wave   = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    out_t* const   vec_data = static_cast<out_t*>(c.data) + c.mis;
-
+    const size_t       wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if(wave * TILE < c.vec_n)
-    {
-        const size_t   v  = wave * TILE + lane;
-        const uint64_t cc = clo + v;
-        W              ctr[DIM];
-        ctr[0] = (W)cc;
-        ctr[1] = (W)(cc >> 32);
-        if constexpr(DIM == 4)
-        {
-            ctr[2] = c2;
-            ctr[3] = c3;
-        }
-        W w[DIM];
-        T::block(ctr, c.key, w);
-        W words[DIM];
-        if constexpr(SUB)
-        {
-            W nx[DIM];
-#pragma unroll
-            for(int i = 0; i < DIM; i++)
-                nx[i] = __shfl_down(w[i], 1, 64);
-            interleave<W, DIM>(w, nx, sub, words);
-        }
-        else
-        {
-#pragma unroll
-            for(int i = 0; i < DIM; i++)
-                words[i] = w[i];
-        }
-        if(v < c.vec_n && (!SUB || lane != 63u))
-        {
-            out_t o[FULL];
-#pragma unroll
-            for(int s = 0; s < OPT; s++)
-                D::apply(words + s * IN, o + s * OUT, c.params);
-            uint32_t u[BYTES / 4];
-            __builtin_memcpy(u, o, BYTES);
-            store_vec<BYTES>(vec_data + v * FULL, u);
-        }
-    }
-
+        fast_tile<K, D, SUB>(c, wave * TILE, c.vec_n);
     if(blockIdx.x == 0 && threadIdx.x == 0)
         head_tail<K, D>(c);
 }
@@ -294,9 +45,7 @@ hipError_t launch_one(const counter_call& c, hipStream_t s)
     {
         // Fast path unless the head shifted some vecs (quirk) or the counter's low
         // 64 bits would carry within the call (4-word engines only).
-        const uint64_t clo     = c.vec_base.c[0] | (c.vec_base.c[1] << 32);
-        const bool     carries = ctr_traits<K>::DIM == 4 && clo + c.vec_n + 1 < clo;
-        if(!c.quirk && !carries)
+        if(fast_ok<K>(c))
         {
             const uint32_t tile   = c.vec_base.sub ? 63u : 64u;
             const size_t   tiles  = (c.vec_n + tile - 1) / tile; // one wavefront per tile

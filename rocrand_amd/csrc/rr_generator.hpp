@@ -107,6 +107,15 @@ protected:
 
 namespace rr
 {
+// One generate call of a fresh counter-based generator (seed, offset), resolved on the
+// host without launching (c.params is left to the caller).
+rocrand_status prepare_counter_call(rocrand_rng_type t, unsigned long long seed, unsigned long long offset,
+                                    out_kind k, void* data, size_t n, counter_call& c, dist_shape& s,
+                                    ctr_kind& kind);
+// Seeded state (before any jump) of an interleaved generator's engines, and its shape for k.
+rocrand_status interleaved_seed_state(rocrand_rng_type t, unsigned long long seed, out_kind k, uint32_t w[6],
+                                      inter_kind& kind, dist_shape& s);
+
 // Factories (rr_api.cpp).
 rocrand_generator make_counter_generator(rocrand_rng_type t);
 rocrand_generator make_interleaved_generator(rocrand_rng_type t);

@@ -12,165 +12,14 @@
 // e subsequences and then by offset/S + [e < offset mod S] steps, with matrix
 // powers (mod-m 3x3 for the MRGs, GF(2) for XORWOW / LFSR113) computed on the
 // host once per device (rr_tables.cpp).
-#include "rr_launch.hpp"
+#include "rr_interleaved_dev.hpp"
 
 namespace rr
 {
 namespace
 {
 
-constexpr uint32_t S = INTERLEAVED_ENGINES;
-
-template<inter_kind K>
-struct ieng;
-
-template<>
-struct ieng<inter_kind::mrg32k3a>
-{
-    static constexpr word_family F = word_family::mrg32;
-    mrg_state                    s;
-    __device__ void load(const uint32_t* st, uint32_t e)
-    {
-#pragma unroll
-        for(int j = 0; j < 3; j++)
-        {
-            s.g1[j] = st[j * S + e];
-            s.g2[j] = st[(3 + j) * S + e];
-        }
-    }
-    __device__ void store(uint32_t* st, uint32_t e) const
-    {
-#pragma unroll
-        for(int j = 0; j < 3; j++)
-        {
-            st[j * S + e]       = s.g1[j];
-            st[(3 + j) * S + e] = s.g2[j];
-        }
-    }
-    __host__ void set(const uint32_t* w)
-    {
-        for(int j = 0; j < 3; j++)
-        {
-            s.g1[j] = w[j];
-            s.g2[j] = w[3 + j];
-        }
-    }
-    __device__ uint32_t next() { return mrg32k3a_next(s); }
-    // Apply X^(sum of set bits) from a table of X^(2^i) (rocrand_mrg32k3a.h:204-263).
-    __device__ void jump(const uint32_t* tab, uint64_t v)
-    {
-        for(int i = 0; v > 0; v >>= 1, i++)
-            if(v & 1)
-            {
-                mat3_vec_mod(tab + (i * 2 + 0) * 9, s.g1, MRG32_M1);
-                mat3_vec_mod(tab + (i * 2 + 1) * 9, s.g2, MRG32_M2);
-            }
-    }
-    __device__ void init(const device_tables& t, uint32_t e, uint64_t off)
-    {
-        jump(t.mrg32_sub, e);
-        jump(t.mrg32_step, off);
-    }
-};
-
-template<>
-struct ieng<inter_kind::mrg31k3p> : ieng<inter_kind::mrg32k3a>
-{
-    static constexpr word_family F = word_family::mrg31;
-    __device__ uint32_t next() { return mrg31k3p_next(s); }
-    __device__ void jump(const uint32_t* tab, uint64_t v)
-    {
-        for(int i = 0; v > 0; v >>= 1, i++)
-            if(v & 1)
-            {
-                mat3_vec_mod(tab + (i * 2 + 0) * 9, s.g1, MRG31_M1);
-                mat3_vec_mod(tab + (i * 2 + 1) * 9, s.g2, MRG31_M2);
-            }
-    }
-    __device__ void init(const device_tables& t, uint32_t e, uint64_t off)
-    {
-        jump(t.mrg31_sub, e);
-        jump(t.mrg31_step, off);
-    }
-};
-
-// Base-4 digit expansion with matrix k = M^(c * 4^k) applied digit times
-// (rocrand_xorwow.h:186-200, rocrand_lfsr113.h jump()).
-template<int N>
-__device__ void gf2_jump(uint32_t* v, const uint32_t* mats, uint64_t n)
-{
-    for(int k = 0; n > 0; n >>= 2, k++)
-    {
-        const uint32_t d = (uint32_t)(n & 3u);
-        for(uint32_t i = 0; i < d; i++)
-            gf2_mat_vec<N>(mats + (size_t)k * 32 * N * N, v);
-    }
-}
-
-template<>
-struct ieng<inter_kind::xorwow>
-{
-    static constexpr word_family F = word_family::w32;
-    xorwow_state                 s;
-    __device__ void load(const uint32_t* st, uint32_t e)
-    {
-        s.d = st[e];
-#pragma unroll
-        for(int j = 0; j < 5; j++)
-            s.x[j] = st[(1 + j) * S + e];
-    }
-    __device__ void store(uint32_t* st, uint32_t e) const
-    {
-        st[e] = s.d;
-#pragma unroll
-        for(int j = 0; j < 5; j++)
-            st[(1 + j) * S + e] = s.x[j];
-    }
-    __host__ void set(const uint32_t* w)
-    {
-        s.d = w[0];
-        for(int j = 0; j < 5; j++)
-            s.x[j] = w[1 + j];
-    }
-    __device__ uint32_t next() { return xorwow_next(s); }
-    __device__ void init(const device_tables& t, uint32_t e, uint64_t off)
-    {
-        gf2_jump<5>(s.x, t.xorwow_seq, e);
-        gf2_jump<5>(s.x, t.xorwow_jump, off);
-        s.d += (uint32_t)off * 362437u; // rocrand_xorwow.h:134-145
-    }
-};
-
-template<>
-struct ieng<inter_kind::lfsr113>
-{
-    static constexpr word_family F = word_family::w32;
-    lfsr113_state                s;
-    __device__ void load(const uint32_t* st, uint32_t e)
-    {
-#pragma unroll
-        for(int j = 0; j < 4; j++)
-            s.z[j] = st[j * S + e];
-    }
-    __device__ void store(uint32_t* st, uint32_t e) const
-    {
-#pragma unroll
-        for(int j = 0; j < 4; j++)
-            st[j * S + e] = s.z[j];
-    }
-    __host__ void set(const uint32_t* w)
-    {
-        for(int j = 0; j < 4; j++)
-            s.z[j] = w[j];
-    }
-    __device__ uint32_t next() { return lfsr113_next(s); }
-    __device__ void init(const device_tables& t, uint32_t e, uint64_t off)
-    {
-        // lfsr113.hpp:45-62: the step count travels as an unsigned int.
-        gf2_jump<4>(s.z, t.lfsr_seq, (uint32_t)e);
-        gf2_jump<4>(s.z, t.lfsr_jump, (uint32_t)off);
-    }
-};
+using namespace inter_dev;
 
 template<inter_kind K>
 __global__ __launch_bounds__(256) void init_kernel(uint32_t* states, const device_tables t, ieng<K> seeded,

@@ -96,8 +96,68 @@ def load():
     L.rocrand_get_direction_vectors64.argtypes = [ctypes.POINTER(ctypes.POINTER(ctypes.c_ulonglong)), ctypes.c_int]
     L.rocrand_get_scramble_constants32.argtypes = [ctypes.POINTER(ctypes.POINTER(ctypes.c_uint))]
     L.rocrand_get_scramble_constants64.argtypes = [ctypes.POINTER(ctypes.POINTER(ctypes.c_ulonglong))]
+    L.rocrand_batch_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(BatchStream), sz, vp]
+    L.rocrand_batch_plan_launch.argtypes = [vp]
+    L.rocrand_batch_plan_destroy.argtypes = [vp]
+    L.rocrand_generate_batched.argtypes = [ctypes.POINTER(BatchStream), sz, vp]
     _lib = L
     return L
+
+
+# rocrand_batched.h: one fresh generator's single generate call per stream
+BATCH_UINT, BATCH_UNIFORM_FLOAT, BATCH_UNIFORM_DOUBLE, BATCH_NORMAL_FLOAT = 0, 1, 2, 3
+BATCH_NORMAL_DOUBLE, BATCH_LOG_NORMAL_FLOAT, BATCH_LOG_NORMAL_DOUBLE, BATCH_POISSON = 4, 5, 6, 7
+
+
+class BatchStream(ctypes.Structure):
+    _fields_ = [("rng_type", ctypes.c_int), ("kind", ctypes.c_int), ("seed", ctypes.c_ulonglong),
+                ("offset", ctypes.c_ulonglong), ("n", ctypes.c_size_t), ("a", ctypes.c_double),
+                ("b", ctypes.c_double), ("output", ctypes.c_void_p)]
+
+
+def _stream_handle(stream):
+    if stream is None:
+        return None
+    return ctypes.c_void_p(getattr(stream, "cuda_stream", stream))
+
+
+def _batch_array(streams):
+    arr = (BatchStream * len(streams))()
+    for i, st in enumerate(streams):
+        arr[i] = st if isinstance(st, BatchStream) else BatchStream(*st)
+    return arr
+
+
+def generate_batched(streams, stream=None):
+    """One-shot batched generation (rocrand_generate_batched); `streams` are BatchStream
+    or (rng_type, kind, seed, offset, n, a, b, output_pointer) tuples."""
+    arr = _batch_array(streams)
+    check_rocrand(load().rocrand_generate_batched(arr, len(streams), _stream_handle(stream)))
+
+
+class BatchPlan:
+    """rocrand_batch_plan: a batch laid out once, replayed as one graph launch."""
+
+    def __init__(self, streams, stream=None):
+        self._arr = _batch_array(streams)
+        self._plan = ctypes.c_void_p()
+        check_rocrand(load().rocrand_batch_plan_create(ctypes.byref(self._plan), self._arr, len(streams),
+                                                       _stream_handle(stream)))
+
+    def launch(self):
+        check_rocrand(load().rocrand_batch_plan_launch(self._plan))
+
+    def close(self):
+        if self._plan:
+            check_rocrand(load().rocrand_batch_plan_destroy(self._plan))
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            if self._plan and _lib is not None:
+                _lib.rocrand_batch_plan_destroy(self._plan)
+        except Exception:
+            pass
 
 
 class RocRandError(Exception):
