@@ -185,7 +185,8 @@ def serving_bench(torch, rates=(1000, 10000, 50000), seconds=1.0, budget_gb=32):
         res = srv.serve(reqs)
         out.append({"rate_rps": rate, "requests": len(reqs), "p50_ms": round(res.percentile(50) * 1e3, 3),
                     "p99_ms": round(res.percentile(99) * 1e3, 3), "GBps": round(res.bytes / res.wall_s / 1e9, 1),
-                    "iterations": res.iterations, "max_batch": res.max_batch, "rejected": res.rejected,
+                    "iterations": res.iterations, "max_batch": res.max_batch, "max_inflight": res.max_inflight,
+                    "rejected": res.rejected,
                     "wall_s": round(res.wall_s, 3)})
         print(f"# serving: {out[-1]}", file=sys.stderr, flush=True)
     del srv

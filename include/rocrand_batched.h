@@ -65,7 +65,8 @@ ROCRANDAPI rocrand_status rocrand_batch_plan_create(rocrand_batch_plan*         
                                                     hipStream_t                 stream);
 /* Writes every stream's output (asynchronously on the plan's stream). */
 ROCRANDAPI rocrand_status rocrand_batch_plan_launch(rocrand_batch_plan plan);
-/* Releases the plan after its work on the plan's stream has finished. */
+/* Releases the plan once its last launch has finished (work queued later on the stream
+ * is not waited for). */
 ROCRANDAPI rocrand_status rocrand_batch_plan_destroy(rocrand_batch_plan plan);
 /* One-shot: plan, launch without a graph, release. */
 ROCRANDAPI rocrand_status rocrand_generate_batched(const rocrand_batch_stream* streams,

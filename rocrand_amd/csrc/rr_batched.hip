@@ -246,9 +246,13 @@ struct rocrand_batch_plan_st
     device_tables              tables{};
     hipGraph_t                 graph = nullptr;
     hipGraphExec_t             exec  = nullptr;
+    hipEvent_t                 done  = nullptr; // recorded after each launch
+    bool                       launched = false;
 
     ~rocrand_batch_plan_st()
     {
+        if(done)
+            (void)hipEventDestroy(done);
         if(exec)
             (void)hipGraphExecDestroy(exec);
         if(graph)
@@ -495,6 +499,8 @@ rocrand_status ROCRANDAPI rocrand_batch_plan_create(rocrand_batch_plan*         
         rocrand_status st = build(*p, streams, count);
         if(st == ROCRAND_STATUS_SUCCESS)
             st = tune(*p);
+        if(st == ROCRAND_STATUS_SUCCESS && hipEventCreateWithFlags(&p->done, hipEventDisableTiming) != hipSuccess)
+            st = ROCRAND_STATUS_INTERNAL_ERROR;
         if(st == ROCRAND_STATUS_SUCCESS)
         {
             // Record the group launches once.  The null stream cannot be captured: record
@@ -545,15 +551,19 @@ rocrand_status ROCRANDAPI rocrand_batch_plan_launch(rocrand_batch_plan plan)
         return ROCRAND_STATUS_NOT_CREATED;
     if(plan->groups.empty())
         return ROCRAND_STATUS_SUCCESS;
-    return hipGraphLaunch(plan->exec, plan->stream) == hipSuccess ? ROCRAND_STATUS_SUCCESS
-                                                                  : ROCRAND_STATUS_LAUNCH_FAILURE;
+    if(hipGraphLaunch(plan->exec, plan->stream) != hipSuccess || hipEventRecord(plan->done, plan->stream) != hipSuccess)
+        return ROCRAND_STATUS_LAUNCH_FAILURE;
+    plan->launched = true;
+    return ROCRAND_STATUS_SUCCESS;
 }
 
 rocrand_status ROCRANDAPI rocrand_batch_plan_destroy(rocrand_batch_plan plan)
 {
     if(plan == nullptr)
         return ROCRAND_STATUS_NOT_CREATED;
-    (void)hipStreamSynchronize(plan->stream);
+    // wait for this plan's last launch only (later work on the stream may still run)
+    if(plan->launched)
+        (void)hipEventSynchronize(plan->done);
     delete plan;
     return ROCRAND_STATUS_SUCCESS;
 }

@@ -1,16 +1,21 @@
 """Continuous-batching request server over the batched C ABI.
 
 Requests (rocrand_amd.workloads.Request) arrive open-loop at their timestamps.  The
-server loop repeatedly admits every arrived request, first come first served, while the
-outputs fit the HBM budget (one pre-allocated arena; a request larger than the whole
-budget is rejected), writes the admitted Philox4x32-10 / MRG32k3a requests with one
-rocrand_generate_batched call and each Sobol64 request with its own quasi-random
-generator call on the same stream, and marks them complete when that stream's work has
-finished.  Latency = completion time - arrival time.
+server loop never waits for the device while work is pending:
 
-A request's output does not depend on what it was batched with: it is the output of
-the plain call sequence of include/rocrand_batched.h (Sobol64: a 16-dimensional
-generator at offset seed mod 2^20).
+* every turn it retires the batches whose completion event has fired (their outputs
+  are done; their memory returns to the arena) and timestamps their requests;
+* it then admits the queued requests, first come first served, whose outputs fit the
+  free arena memory (one pre-allocated HBM budget, first-fit with coalescing; a request
+  larger than the whole budget is rejected), and issues them as one batch: one batch
+  plan (include/rocrand_batched.h) for the Philox4x32-10 / MRG32k3a requests, one
+  quasi-random generator call per Sobol64 request, then a completion event;
+* consecutive batches alternate between two device streams, so a batch admitted while
+  earlier ones run overlaps with them on the device.
+
+Latency = time the completion was observed - arrival time.  A request's output does
+not depend on its batch: it is the output of the plain call sequence of
+include/rocrand_batched.h (Sobol64: a 16-dimensional generator at offset seed mod 2^20).
 """
 from __future__ import annotations
 
@@ -21,6 +26,7 @@ from . import rocrand as R
 
 KIND = {"f32": R.BATCH_UNIFORM_FLOAT, "f64": R.BATCH_UNIFORM_DOUBLE, "u32": R.BATCH_UINT}
 ITEMSIZE = {"f32": 4, "f64": 8, "u32": 4}
+ALIGN = 256
 
 
 @dataclass
@@ -31,6 +37,7 @@ class Result:
     iterations: int = 0
     rejected: int = 0
     max_batch: int = 0
+    max_inflight: int = 0
 
     def percentile(self, q):
         if not self.latencies_s:
@@ -42,6 +49,10 @@ class Result:
 
 def request_bytes(req) -> int:
     return req.n * ITEMSIZE[req.dtype]
+
+
+def padded(nbytes: int) -> int:
+    return (nbytes + ALIGN - 1) // ALIGN * ALIGN
 
 
 def sobol_offset(req) -> int:
@@ -66,78 +77,133 @@ def write_sobol(req, view, stream):
         g.uniform(view)
 
 
-class Server:
-    """One stream, one arena of `budget_bytes` device memory."""
+class Arena:
+    """First-fit allocator over [0, size) with coalescing of freed ranges."""
 
-    def __init__(self, torch, budget_bytes: int, stream=None):
+    def __init__(self, size: int):
+        self.size = size
+        self.free = [(0, size)]  # sorted (start, length)
+
+    def alloc(self, n: int):
+        for i, (s, ln) in enumerate(self.free):
+            if ln >= n:
+                if ln == n:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (s + n, ln - n)
+                return s
+        return None
+
+    def release(self, start: int, n: int):
+        self.free.append((start, n))
+        self.free.sort()
+        merged = []
+        for s, ln in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == s:
+                merged[-1] = (merged[-1][0], merged[-1][1] + ln)
+            else:
+                merged.append((s, ln))
+        self.free = merged
+
+
+@dataclass
+class _Inflight:
+    reqs: list
+    event: object
+    plan: object
+    regions: list
+
+
+class Server:
+    def __init__(self, torch, budget_bytes: int, streams: int = 2):
         self.torch = torch
         self.budget = budget_bytes
-        self.arena = torch.empty(budget_bytes, dtype=torch.uint8, device="cuda")
-        self.stream = stream or torch.cuda.Stream()
+        self.memory = torch.empty(budget_bytes, dtype=torch.uint8, device="cuda")
+        self.arena = Arena(budget_bytes)
+        self.streams = [torch.cuda.Stream() for _ in range(streams)]
+        self.turn = 0
 
-    def views(self, reqs):
-        """Arena slices (256-byte aligned) for the admitted requests."""
+    def _view(self, start, req):
+        dt = {"f32": self.torch.float32, "f64": self.torch.float64, "u32": self.torch.uint32}[req.dtype]
+        return self.memory[start:start + request_bytes(req)].view(dt)
+
+    def issue(self, reqs, regions):
+        """Queue one batch on the next stream; returns (event, plan, views)."""
         torch = self.torch
-        dt = {"f32": torch.float32, "f64": torch.float64, "u32": torch.uint32}
-        pos, out = 0, []
-        for r in reqs:
-            nb = request_bytes(r)
-            v = self.arena[pos:pos + nb].view(dt[r.dtype])
-            out.append(v)
-            pos += (nb + 255) // 256 * 256
-        return out
-
-    def admit(self, queue, start):
-        """FIFO admission while the 256-byte-rounded outputs fit the budget."""
-        used, k = 0, start
-        while k < len(queue):
-            nb = (request_bytes(queue[k]) + 255) // 256 * 256
-            if used + nb > self.budget:
-                break
-            used += nb
-            k += 1
-        return k
+        stream = self.streams[self.turn % len(self.streams)]
+        self.turn += 1
+        views = [self._view(s, r) for s, r in zip(regions, reqs)]
+        plan = None
+        streams = batch_streams(reqs, [v.data_ptr() for v in views])
+        with torch.cuda.stream(stream):
+            if streams:
+                plan = R.BatchPlan(streams, stream)
+                plan.launch()
+            for r, v in zip(reqs, views):
+                if r.rng_type == R.SOBOL64:
+                    write_sobol(r, v, stream)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        return ev, plan, views
 
     def run_batch(self, reqs):
-        views = self.views(reqs)
-        ptrs = [v.data_ptr() for v in views]
-        streams = batch_streams(reqs, ptrs)
-        if streams:
-            R.generate_batched(streams, self.stream)
-        for r, v in zip(reqs, views):
-            if r.rng_type == R.SOBOL64:
-                write_sobol(r, v, self.stream)
+        """Writes `reqs` as one batch and waits for it (outputs stay valid until the next call)."""
+        self.arena = Arena(self.budget)
+        regions = [self.arena.alloc(padded(request_bytes(r))) for r in reqs]
+        if any(s is None for s in regions):
+            raise ValueError("batch exceeds the budget")
+        ev, plan, views = self.issue(reqs, regions)
+        ev.synchronize()
+        if plan is not None:
+            plan.close()
+        self.arena = Arena(self.budget)
         return views
 
     def serve(self, requests) -> Result:
         """Open-loop run: request i becomes visible at its arrival time."""
-        torch = self.torch
         res = Result()
         i = 0
+        queue, inflight = [], []
         t0 = time.perf_counter()
-        queue = []
-        while i < len(requests) or queue:
+        while i < len(requests) or queue or inflight:
             now = time.perf_counter() - t0
+            # retire finished batches
+            still = []
+            for b in inflight:
+                if b.event.query():
+                    done = time.perf_counter() - t0
+                    for r in b.reqs:
+                        res.latencies_s.append(done - r.arrival_s)
+                        res.bytes += request_bytes(r)
+                    if b.plan is not None:
+                        b.plan.close()
+                    for s, r in zip(b.regions, b.reqs):
+                        self.arena.release(s, padded(request_bytes(r)))
+                else:
+                    still.append(b)
+            inflight = still
+            # arrivals
             while i < len(requests) and requests[i].arrival_s <= now:
-                if request_bytes(requests[i]) > self.budget:
+                if padded(request_bytes(requests[i])) > self.budget:
                     res.rejected += 1
                 else:
                     queue.append(requests[i])
                 i += 1
-            if not queue:
-                if i < len(requests):
-                    time.sleep(max(0.0, min(1e-3, requests[i].arrival_s - now)))
-                continue
-            k = self.admit(queue, 0)
-            batch, queue = queue[:k], queue[k:]
-            with torch.cuda.stream(self.stream):
-                self.run_batch(batch)
-            self.stream.synchronize()
-            done = time.perf_counter() - t0
-            for r in batch:
-                res.latencies_s.append(done - r.arrival_s)
-                res.bytes += request_bytes(r)
-            res.iterations += 1
-            res.max_batch = max(res.max_batch, len(batch))
+            # admission: FIFO while memory is free
+            batch, regions = [], []
+            while queue:
+                s = self.arena.alloc(padded(request_bytes(queue[0])))
+                if s is None:
+                    break
+                batch.append(queue.pop(0))
+                regions.append(s)
+            if batch:
+                ev, plan, _ = self.issue(batch, regions)
+                inflight.append(_Inflight(batch, ev, plan, regions))
+                res.iterations += 1
+                res.max_batch = max(res.max_batch, len(batch))
+                res.max_inflight = max(res.max_inflight, len(inflight))
+            elif not inflight and i < len(requests):
+                time.sleep(max(0.0, min(2e-4, requests[i].arrival_s - now)))
         res.wall_s = time.perf_counter() - t0
         return res
