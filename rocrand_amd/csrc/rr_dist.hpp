@@ -36,15 +36,125 @@ constexpr float  R_2PI            = 6.2831855f;
 constexpr float  R_SQRT2          = 1.4142135f;
 
 // ---------------------------------------------------------------- scalar math
-// Float transcendentals evaluated in double and rounded once: correctly rounded
-// except in cases within 2^-29 relative of a rounding boundary, so each differs
-// from the host libm's float result by at most the host's own error (< 1 ulp).
-__device__ __forceinline__ float logf_acc(float x) { return (float)log((double)x); }
-__device__ __forceinline__ float expf_acc(float x) { return (float)exp((double)x); }
-__device__ __forceinline__ void  sincosf_acc(float x, float* s, float* c)
+// Float transcendentals evaluated in double (relative error < 2^-50) and rounded once:
+// correctly rounded except where the exact value lies within 2^-50 of a float rounding
+// boundary, so each differs from the host libm's float result by at most the host's own
+// error (< 1 ulp).  Written for the arguments the transforms produce, with short fixed
+// polynomials instead of the general-purpose library routines:
+//   log: x = 2^e m, m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh(f), f = (m-1)/(m+1), |f| < 0.172,
+//        series to f^21; checked equal to the correctly rounded float log for every float
+//        in [2^-32, 1];
+//   sin/cos for x in [0, 8): Cody-Waite reduction by pi/2 (two parts, fused), Taylor
+//        series to r^15 / r^16 on |r| <= pi/4; equal to the correctly rounded results for
+//        every float in [0, 7];
+//   exp: reduction by ln 2 (two parts), Taylor series to r^13 on |r| <= 0.347, scaling by
+//        2^k in double; equal to the correctly rounded float exp on a 1/37 sample of all
+//        floats.
+constexpr double RR_LN2_HI  = 6.93147180369123816490e-01; // ln 2, top 32 bits
+constexpr double RR_LN2_LO  = 1.90821492927058770002e-10; // ln 2 - RR_LN2_HI
+constexpr double RR_PIO2_1  = 1.57079632679489655800e+00; // pi/2 rounded to double
+constexpr double RR_PIO2_2  = 6.12323399573676603587e-17; // pi/2 - RR_PIO2_1
+constexpr double RR_2_OV_PI = 6.36619772367581382433e-01;
+
+__device__ __forceinline__ double log_dbl(double x)
+{
+    int    e = __builtin_amdgcn_frexp_exp(x);
+    double m = __builtin_amdgcn_frexp_mant(x); // [0.5, 1)
+    if(m < 0.70710678118654752440)
+    {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    const double d  = m + 1.0;
+    double       r  = __builtin_amdgcn_rcp(d);
+    r               = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+    const double f  = (m - 1.0) * r;
+    const double f2 = f * f;
+    double       p  = 1.0 / 21;
+    p               = __builtin_fma(p, f2, 1.0 / 19);
+    p               = __builtin_fma(p, f2, 1.0 / 17);
+    p               = __builtin_fma(p, f2, 1.0 / 15);
+    p               = __builtin_fma(p, f2, 1.0 / 13);
+    p               = __builtin_fma(p, f2, 1.0 / 11);
+    p               = __builtin_fma(p, f2, 1.0 / 9);
+    p               = __builtin_fma(p, f2, 1.0 / 7);
+    p               = __builtin_fma(p, f2, 1.0 / 5);
+    p               = __builtin_fma(p, f2, 1.0 / 3);
+    const double tf = 2.0 * f;
+    const double lm = __builtin_fma(tf * f2, p, tf);
+    const double y  = __builtin_fma((double)e, RR_LN2_HI, __builtin_fma((double)e, RR_LN2_LO, lm));
+    if(x == 0.0)
+        return -__builtin_inf();
+    if(!(x > 0.0) || x == __builtin_inf())
+        return x < 0.0 ? __builtin_nan("") : x; // NaN, negative -> NaN; +inf -> +inf
+    return y;
+}
+
+__device__ __forceinline__ void sincos_dbl(double x, double* s, double* c)
+{
+    const double q  = __builtin_rint(x * RR_2_OV_PI);
+    double       r  = __builtin_fma(-q, RR_PIO2_1, x);
+    r               = __builtin_fma(-q, RR_PIO2_2, r);
+    const double r2 = r * r;
+    double       ps = -1.0 / 1307674368000.0;
+    ps              = __builtin_fma(ps, r2, 1.0 / 6227020800.0);
+    ps              = __builtin_fma(ps, r2, -1.0 / 39916800.0);
+    ps              = __builtin_fma(ps, r2, 1.0 / 362880.0);
+    ps              = __builtin_fma(ps, r2, -1.0 / 5040.0);
+    ps              = __builtin_fma(ps, r2, 1.0 / 120.0);
+    ps              = __builtin_fma(ps, r2, -1.0 / 6.0);
+    const double sn = __builtin_fma(r * r2, ps, r);
+    double       pc = 1.0 / 20922789888000.0;
+    pc              = __builtin_fma(pc, r2, -1.0 / 87178291200.0);
+    pc              = __builtin_fma(pc, r2, 1.0 / 479001600.0);
+    pc              = __builtin_fma(pc, r2, -1.0 / 3628800.0);
+    pc              = __builtin_fma(pc, r2, 1.0 / 40320.0);
+    pc              = __builtin_fma(pc, r2, -1.0 / 720.0);
+    pc              = __builtin_fma(pc, r2, 1.0 / 24.0);
+    pc              = __builtin_fma(pc, r2, -0.5);
+    const double cs = __builtin_fma(r2, pc, 1.0);
+    const int    qi = (int)q & 3;
+    const double a  = (qi & 1) ? cs : sn;
+    const double b  = (qi & 1) ? sn : cs;
+    *s              = (qi & 2) ? -a : a;
+    *c              = ((qi + 1) & 2) ? -b : b;
+}
+
+__device__ __forceinline__ double exp_dbl(double x)
+{
+    const double xc = x < -110.0 ? -110.0 : (x > 100.0 ? 100.0 : x);
+    const double k  = __builtin_rint(xc * 1.44269504088896338700);
+    double       r  = __builtin_fma(-k, RR_LN2_HI, xc);
+    r               = __builtin_fma(-k, RR_LN2_LO, r);
+    double p = 1.0 / 6227020800.0;
+    p        = __builtin_fma(p, r, 1.0 / 479001600.0);
+    p        = __builtin_fma(p, r, 1.0 / 39916800.0);
+    p        = __builtin_fma(p, r, 1.0 / 3628800.0);
+    p        = __builtin_fma(p, r, 1.0 / 362880.0);
+    p        = __builtin_fma(p, r, 1.0 / 40320.0);
+    p        = __builtin_fma(p, r, 1.0 / 5040.0);
+    p        = __builtin_fma(p, r, 1.0 / 720.0);
+    p        = __builtin_fma(p, r, 1.0 / 120.0);
+    p        = __builtin_fma(p, r, 1.0 / 24.0);
+    p        = __builtin_fma(p, r, 1.0 / 6.0);
+    p        = __builtin_fma(p, r, 0.5);
+    p        = __builtin_fma(p, r, 1.0);
+    p        = __builtin_fma(p, r, 1.0);
+    double y = __builtin_amdgcn_ldexp(p, (int)k);
+    if(x < -110.0)
+        y = 0.0;
+    if(x > 100.0)
+        y = __builtin_inf();
+    return x != x ? x : y;
+}
+
+__device__ __forceinline__ float logf_acc(float x) { return (float)log_dbl((double)x); }
+__device__ __forceinline__ float expf_acc(float x) { return (float)exp_dbl((double)x); }
+// x in [0, 8): the Box-Muller angles
+__device__ __forceinline__ void sincosf_acc(float x, float* s, float* c)
 {
     double sd, cd;
-    sincos((double)x, &sd, &cd);
+    sincos_dbl((double)x, &sd, &cd);
     *s = (float)sd;
     *c = (float)cd;
 }

@@ -101,9 +101,108 @@ __global__ __launch_bounds__(256) void generate_kernel(interleaved_call c)
     g.store(c.states, eid);
 }
 
+// MRG engines: each engine's vec-steps are split into J chunks of C, one lane per
+// (engine, chunk).  A lane jumps its copy of the engine state ahead by j*C*IN words
+// with the A^(2^i) tables (a handful of 3x3 modular products), so S*J lanes run the
+// serial recurrence instead of S: the per-engine dependency chain no longer limits the
+// store rate.  Lanes are ordered chunk-major, so a wavefront's 64 lanes are 64
+// consecutive engines of one chunk and each of its stores is one contiguous segment.
+// The lane whose range ends at the engine's last vec-step also writes the head/tail
+// (for the engine that produces vec index vec_n) and stores the final state.
+template<inter_kind K, class D>
+__global__ __launch_bounds__(256) void generate_chunked_kernel(interleaved_call c, const device_tables t,
+                                                               uint32_t J, uint32_t C)
+{
+    using out_t         = typename D::out_t;
+    constexpr int IN    = D::in_w;
+    constexpr int OUT   = D::out_w;
+    constexpr int BYTES = OUT * (int)sizeof(out_t);
+    static_assert(BYTES == 4 || BYTES == 8 || BYTES == 16, "vec size");
+
+    const uint32_t g  = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t id = g % S, j = g / S;
+    if(j >= J)
+        return;
+    // vec-steps of this engine: vec indices id, id + S, ... below vec_n
+    const uint64_t steps  = id < c.vec_n ? (c.vec_n - id + S - 1) / S : 0;
+    const uint64_t k0     = (uint64_t)j * C;
+    const uint64_t end_j  = steps / C < J ? steps / C : J - 1; // chunk that ends at `steps`
+    const bool     ending = j == end_j;
+    if(k0 > steps && !ending)
+        return;
+    const uint32_t eid = (id + c.start) % S;
+    ieng<K>        eng;
+    eng.load(c.states, eid);
+    eng.jump(K == inter_kind::mrg32k3a ? t.mrg32_step : t.mrg31_step, k0 * IN);
+
+    out_t* const data     = static_cast<out_t*>(c.data);
+    out_t* const vec_data = data + c.mis;
+    alignas(16) out_t o[OUT];
+    uint32_t     in[IN];
+    const uint64_t k1 = ending ? steps : (k0 + C < steps ? k0 + C : steps);
+    for(uint64_t k = k0; k < k1; k++)
+    {
+#pragma unroll
+        for(int i = 0; i < IN; i++)
+            in[i] = eng.next();
+        D::apply(in, o, c.params);
+        const size_t index = id + k * S;
+        if constexpr(BYTES == 4)
+            *reinterpret_cast<uint32_t*>(vec_data + index * OUT) = *reinterpret_cast<const uint32_t*>(o);
+        else if constexpr(BYTES == 8)
+            *reinterpret_cast<uint2*>(vec_data + index * OUT) = *reinterpret_cast<const uint2*>(o);
+        else
+            *reinterpret_cast<uint4*>(vec_data + index * OUT) = *reinterpret_cast<const uint4*>(o);
+    }
+    if(!ending)
+        return;
+    if(OUT > 1 && id + steps * S == c.vec_n)
+    {
+        if(c.head > 0)
+        {
+#pragma unroll
+            for(int i = 0; i < IN; i++)
+                in[i] = eng.next();
+            D::apply(in, o, c.params);
+            for(uint32_t q = 0; q < (uint32_t)OUT; q++)
+                if(q < c.head)
+                    data[q] = o[q];
+        }
+        if(c.tail > 0)
+        {
+#pragma unroll
+            for(int i = 0; i < IN; i++)
+                in[i] = eng.next();
+            D::apply(in, o, c.params);
+            for(uint32_t q = 0; q < (uint32_t)OUT; q++)
+                if(q < c.tail)
+                    data[c.n - c.tail + q] = o[q];
+        }
+    }
+    eng.store(c.states, eid);
+}
+
 template<inter_kind K, class D>
 hipError_t launch_gen(const interleaved_call& c, hipStream_t s)
 {
+    if constexpr(K == inter_kind::mrg32k3a || K == inter_kind::mrg31k3p)
+    {
+        // chunks per engine: enough lanes to hide the recurrence's latency
+        const uint64_t kmax = (c.vec_n + S - 1) / S;
+        uint32_t       J    = 1;
+        while(J < 8 && (uint64_t)J * 64 < kmax)
+            J *= 2;
+        if(J > 1)
+        {
+            device_tables t;
+            hipError_t    err = get_device_tables(t);
+            if(err != hipSuccess)
+                return err;
+            const uint32_t C = (uint32_t)((kmax + J - 1) / J);
+            hipLaunchKernelGGL((generate_chunked_kernel<K, D>), dim3(S / 256 * J), dim3(256), 0, s, c, t, J, C);
+            return hipGetLastError();
+        }
+    }
     hipLaunchKernelGGL((generate_kernel<K, D>), dim3(S / 256), dim3(256), 0, s, c);
     return hipGetLastError();
 }

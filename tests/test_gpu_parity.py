@@ -156,3 +156,22 @@ def test_orderings_same_stream(ename, order):
     og = O.OracleGen(rng, seed=5)
     st, want = og.generate("u32", 4099, got[0][1])
     assert np.array_equal(got[0][0], want)
+
+
+@pytest.mark.parametrize("ename", ["mrg32k3a", "mrg31k3p"])
+def test_mrg_large_calls(ename):
+    """Calls long enough (more than 64 vec-steps per engine) to take the chunked MRG kernel,
+    misaligned and ragged, two per generator, against the oracle."""
+    rng = ENGINES[ename]
+    for kind, n, shift in [("u32", (1 << 24) + 3, 1), ("uniform_f32", 1 << 25, 0), ("normal_f32", (1 << 24) + 1, 1),
+                           ("uniform_f64", (1 << 23) + 5, 1)]:
+        print(f"case {ename} {kind} n={n} shift={shift}", flush=True)
+        got = _gpu_stream(rng, kind, 2024, 1048583, n, shift)
+        og = O.OracleGen(rng, seed=2024, offset=1048583)
+        a = KINDS[kind][2]
+        a0 = a[0] if len(a) > 0 else 0.0
+        a1 = a[1] if len(a) > 1 else 0.0
+        for g, addr in got:
+            st, want = og.generate(kind, n, addr, a0, a1)
+            assert st == 0
+            _check(kind, g, want, (a0, a1))
