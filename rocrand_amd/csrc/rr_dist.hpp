@@ -42,14 +42,14 @@ constexpr float  R_SQRT2          = 1.4142135f;
 // error (< 1 ulp).  Written for the arguments the transforms produce, with short fixed
 // polynomials instead of the general-purpose library routines:
 //   log: x = 2^e m, m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh(f), f = (m-1)/(m+1), |f| < 0.172,
-//        series to f^21; checked equal to the correctly rounded float log for every float
-//        in [2^-32, 1];
+//        series to f^21;
 //   sin/cos for x in [0, 8): Cody-Waite reduction by pi/2 (two parts, fused), Taylor
-//        series to r^15 / r^16 on |r| <= pi/4; equal to the correctly rounded results for
-//        every float in [0, 7];
+//        series to r^15 / r^16 on |r| <= pi/4;
 //   exp: reduction by ln 2 (two parts), Taylor series to r^13 on |r| <= 0.347, scaling by
-//        2^k in double; equal to the correctly rounded float exp on a 1/37 sample of all
-//        floats.
+//        2^k in double.
+// A host C twin of these routines (exact division in place of the refined reciprocal)
+// rounds to the correctly rounded float result for every float in [2^-32, 1] (log), every
+// float in [0, 7] (sin, cos) and a 1/37 sample of all floats (exp).
 constexpr double RR_LN2_HI  = 6.93147180369123816490e-01; // ln 2, top 32 bits
 constexpr double RR_LN2_LO  = 1.90821492927058770002e-10; // ln 2 - RR_LN2_HI
 constexpr double RR_PIO2_1  = 1.57079632679489655800e+00; // pi/2 rounded to double
