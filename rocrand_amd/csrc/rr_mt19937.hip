@@ -19,10 +19,11 @@
 //     recurrence steps, since a step reads 397 words ahead) and then gains
 //     XOR_{set k} A^k s, whose word i is S[i + k] for the start state's sequence S
 //     extended by 31 words.  Bit-identical to the one-coefficient-per-step evaluation.
-//   * generate: one block per 8 generators keeps their 8 x 624 words in LDS for the whole
-//     call; a round twists them in three barrier-separated phases ([0,227) reads only old
-//     words, [227,454) and [454,624) read the words 227 before them, already new); each
-//     wavefront then writes 64 consecutive inputs (8 generators x 8 lanes) per local word.
+//   * generate: one block per 8 generators keeps their 8 x 624 words in LDS (element-major)
+//     for the whole call; a round twists them in three barrier-separated phases ([0,227)
+//     reads only old words, [227,454) and [454,624) read the words 227 before them, already
+//     new); each wavefront then writes 64 consecutive inputs (8 generators x 8 lanes) per
+//     local word.
 #include "rr_generator.hpp"
 
 #include <rocrand.h>
@@ -157,8 +158,13 @@ __device__ __forceinline__ unsigned mt_element(unsigned lane, unsigned local)
     return mt_offsets[a] + 7 * lane + b;
 }
 
+// LDS layout of the block's generators: element-major, word k of generator g at
+// st[k * GPB + g].  A wavefront's output reads (8 generators x 8 lanes, lanes 7 words
+// apart) then fall on 64 distinct banks, and the twist's reads are consecutive.
+__device__ __forceinline__ unsigned lds_at(unsigned k, unsigned g) { return k * GPB + g; }
+
 // One round of the recurrence on GPB generators held in LDS.
-__device__ __forceinline__ void twist_block(uint32_t (*st)[MT_N], unsigned t)
+__device__ __forceinline__ void twist_block(uint32_t* st, unsigned t)
 {
     constexpr unsigned PER = (GPB * 227 + 255) / 256; // items per thread per phase
     uint32_t           v[PER];
@@ -176,8 +182,8 @@ __device__ __forceinline__ void twist_block(uint32_t (*st)[MT_N], unsigned t)
             const unsigned it = t + 256 * r;
             if(it < items)
             {
-                const unsigned g = it / len[ph], k = lo[ph] + it % len[ph];
-                v[r] = mt_comp(st[g][k], st[g][wrap(k + 1)], st[g][wrap(k + MT_M)]);
+                const unsigned g = it % GPB, k = lo[ph] + it / GPB;
+                v[r] = mt_comp(st[lds_at(k, g)], st[lds_at(wrap(k + 1), g)], st[lds_at(wrap(k + MT_M), g)]);
             }
         }
         __syncthreads();
@@ -187,8 +193,8 @@ __device__ __forceinline__ void twist_block(uint32_t (*st)[MT_N], unsigned t)
             const unsigned it = t + 256 * r;
             if(it < items)
             {
-                const unsigned g = it / len[ph], k = lo[ph] + it % len[ph];
-                st[g][k]         = v[r];
+                const unsigned g = it % GPB, k = lo[ph] + it / GPB;
+                st[lds_at(k, g)] = v[r];
             }
         }
         __syncthreads();
@@ -206,11 +212,11 @@ __global__ __launch_bounds__(256) void mt_generate_kernel(mt_call c)
     constexpr uint64_t FULL = (uint64_t)MT_STRIDE * IPS;
     static_assert(BYTES == 4 || BYTES == 8 || BYTES == 16, "vec size");
 
-    __shared__ uint32_t st[GPB][MT_N];
+    __shared__ uint32_t st[GPB * MT_N];
     const unsigned t = threadIdx.x, blk = blockIdx.x;
     uint32_t*      gstate = c.states + (size_t)blk * GPB * MT_N;
     for(unsigned i = t; i < GPB * MT_N; i += 256)
-        st[i / MT_N][i % MT_N] = gstate[i];
+        st[lds_at(i % MT_N, i / MT_N)] = gstate[i];
     __syncthreads();
 
     const unsigned p = t & 63u, quarter = t >> 6;
@@ -235,7 +241,7 @@ __global__ __launch_bounds__(256) void mt_generate_kernel(mt_call c)
             const uint64_t v = Q - c.start;
 #pragma unroll
             for(int i = 0; i < IN; i++)
-                in[i] = mt_temper(st[gl][mt_element(lane, j * IN + i)]);
+                in[i] = mt_temper(st[lds_at(mt_element(lane, j * IN + i), gl)]);
             D::apply(in, o, c.params);
             if(v < c.vec_n)
             {
@@ -261,7 +267,7 @@ __global__ __launch_bounds__(256) void mt_generate_kernel(mt_call c)
         __syncthreads();
     }
     for(unsigned i = t; i < GPB * MT_N; i += 256)
-        gstate[i] = st[i / MT_N][i % MT_N];
+        gstate[i] = st[lds_at(i % MT_N, i / MT_N)];
 }
 
 template<class D>
