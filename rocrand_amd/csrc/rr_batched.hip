@@ -7,10 +7,11 @@
 //     the per-call kernels do (rr_counter_dev.hpp), so outputs match the plain path bit
 //     for bit; the stream's head and tail go to its first tile;
 //   * MRG32k3a: a tile is up to 256 of a stream's engines.  A lane initialises its engine
-//     from the stream's seeded state (jump by the engine index in subsequences and by
-//     offset / S + [engine < offset mod S] steps, as the per-call init kernel does) and
-//     then writes that engine's vecs v = id, id + S, ... (rr_interleaved_dev.hpp).  Only
-//     the engines a stream uses are set up: min(S, vec_n + 1) instead of S per call.
+//     (the stream's seeded state advanced by offset / S + [engine < offset mod S] steps --
+//     done once per stream on the host -- then jumped by the engine index in
+//     subsequences, as the per-call init kernel does) and then writes that engine's vecs
+//     v = id, id + S, ... (rr_interleaved_dev.hpp).  Only the engines a stream uses are set
+//     up: min(S, vec_n + 1) instead of S per call.
 // The plan records the group launches in a HIP graph once; a launch replays it.  The
 // grid size of each group is chosen by timing a few candidates on the first plan of a
 // given shape (kept in a process-wide cache).
@@ -42,12 +43,14 @@ struct batch_tile
     uint64_t first;
 };
 
-// One MRG32k3a stream: a fresh generator's single call.
+// One MRG32k3a stream: a fresh generator's single call.  Engine e's state is
+// A^(e * 2^76) A^(offset/S + [e < offset mod S]) (seeded state); the step part is the
+// same for every engine, so it is applied once on the host: base[0] = A^(offset/S) seeded,
+// base[1] = A base[0], and a lane only jumps by its subsequence.
 struct inter_call
 {
-    uint32_t    seeded[6];
+    uint32_t    base[2][6];
     uint32_t    start;    // offset mod S
-    uint64_t    base_off; // offset / S
     void*       data;
     size_t      n, mis, vec_n;
     uint32_t    head, tail;
@@ -107,8 +110,8 @@ __global__ __launch_bounds__(256) void batch_inter_kernel(const inter_call* call
         const inter_call& c   = calls[tl.stream];
         const uint32_t    eid = (id + c.start) % S;
         ieng<K>           g;
-        g.set(c.seeded);
-        g.init(t, eid, c.base_off + (eid < c.start ? 1u : 0u));
+        g.set(c.base[eid < c.start ? 1 : 0]);
+        g.jump(t.mrg32_sub, eid);
         out_t* const data     = static_cast<out_t*>(c.data);
         out_t* const vec_data = data + c.mis;
         alignas(16) out_t o[OUT];
@@ -383,12 +386,25 @@ rocrand_status build(rocrand_batch_plan_st& p, const rocrand_batch_stream* strea
         {
             inter_call     c{};
             inter_kind     ik;
-            rocrand_status st = interleaved_seed_state(b.rng_type, b.seed, k, c.seeded, ik, s);
+            uint32_t       seeded[6];
+            rocrand_status st = interleaved_seed_state(b.rng_type, b.seed, k, seeded, ik, s);
             if(st != ROCRAND_STATUS_SUCCESS)
                 return st;
+            // base[0] = A^(offset / S) seeded, base[1] = A base[0] (host step tables A^(2^i))
+            const uint32_t* steps = host_mrg_table(32, 0);
+            uint64_t        v     = b.offset / INTERLEAVED_ENGINES;
+            for(int i = 0; v > 0; v >>= 1, i++)
+                if(v & 1)
+                {
+                    mat3_vec_fold<32, 209>(steps + (i * 2 + 0) * 9, seeded);
+                    mat3_vec_fold<32, 22853>(steps + (i * 2 + 1) * 9, seeded + 3);
+                }
+            std::copy(seeded, seeded + 6, c.base[0]);
+            mat3_vec_fold<32, 209>(steps + 0 * 9, seeded);
+            mat3_vec_fold<32, 22853>(steps + 1 * 9, seeded + 3);
+            std::copy(seeded, seeded + 6, c.base[1]);
             const uintptr_t addr = reinterpret_cast<uintptr_t>(b.output);
             c.start    = (uint32_t)(b.offset % INTERLEAVED_ENGINES);
-            c.base_off = b.offset / INTERLEAVED_ENGINES;
             c.data     = b.output;
             c.n        = b.n;
             c.mis      = (s.out_w - (addr / s.esz) % s.out_w) % s.out_w;

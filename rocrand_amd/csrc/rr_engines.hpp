@@ -333,13 +333,28 @@ __host__ __device__ __forceinline__ uint32_t mrg32_reduce_m2(uint64_t p)
     return (uint32_t)(p >= MRG32_M2 ? p - MRG32_M2 : p);
 }
 
+// x mod m for x < 2m + 2^32 and m = 2^32 - c, from the 64-bit value's words: x - m is
+// lo + c (mod 2^32) exactly when x >= m, i.e. when hi != 0 or lo >= m.
+__host__ __device__ __forceinline__ uint32_t mrg32_fold_final(uint64_t x, uint32_t c)
+{
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    return lo + ((hi != 0u || lo >= 0u - c) ? c : 0u);
+}
+
 __host__ __device__ __forceinline__ uint32_t mrg32k3a_next(mrg_state& s)
 {
-    const uint32_t p1 = mrg32_reduce_m1(MRG32_A12 * s.g1[1] + MRG32_A13N * (MRG32_M1 - s.g1[0]));
+    // products below 2^53; one fold by 2^32 = 209 (mod m1) leaves x < 2 m1, two folds by
+    // 2^32 = 22853 (mod m2) leave x < 2 m2: the same residues as the reference's reductions
+    const uint64_t t1 = (uint64_t)(uint32_t)MRG32_A12 * s.g1[1]
+                        + (uint64_t)(uint32_t)MRG32_A13N * ((uint32_t)MRG32_M1 - s.g1[0]);
+    const uint32_t p1 = mrg32_fold_final(209ull * (t1 >> 32) + (uint32_t)t1, 209u);
     s.g1[0]           = s.g1[1];
     s.g1[1]           = s.g1[2];
     s.g1[2]           = p1;
-    const uint32_t p2 = mrg32_reduce_m2(MRG32_A21 * s.g2[2] + MRG32_A23N * (MRG32_M2 - s.g2[0]));
+    const uint64_t t2 = (uint64_t)(uint32_t)MRG32_A21 * s.g2[2]
+                        + (uint64_t)(uint32_t)MRG32_A23N * ((uint32_t)MRG32_M2 - s.g2[0]);
+    const uint64_t f2 = 22853ull * (t2 >> 32) + (uint32_t)t2;
+    const uint32_t p2 = mrg32_fold_final(22853ull * (f2 >> 32) + (uint32_t)f2, 22853u);
     s.g2[0]           = s.g2[1];
     s.g2[1]           = s.g2[2];
     s.g2[2]           = p2;
@@ -367,6 +382,38 @@ __host__ __device__ __forceinline__ uint32_t mrg31k3p_next(mrg_state& s)
     s.g2[1] = s.g2[0];
     s.g2[0] = tmp;
     return s.g1[0] - s.g2[0] + (s.g1[0] <= s.g2[0] ? MRG31_M1 : 0u);
+}
+
+// x mod (2^K - C) for any 64-bit x: three folds by 2^K = C leave x < 2m for the four
+// moduli used here (m1, m2 of MRG32k3a and MRG31k3p), then one conditional subtraction.
+template<uint32_t K, uint32_t C>
+__host__ __device__ __forceinline__ uint32_t fold_mod(uint64_t x)
+{
+    constexpr uint64_t mask = (1ull << K) - 1;
+    constexpr uint64_t m    = (1ull << K) - C;
+    x = (x >> K) * C + (x & mask);
+    x = (x >> K) * C + (x & mask);
+    x = (x >> K) * C + (x & mask);
+    return (uint32_t)(x >= m ? x - m : x);
+}
+
+// 3x3 matrix (row-major, entries < m) times state vector, mod m = 2^K - C.
+template<uint32_t K, uint32_t C>
+__host__ __device__ __forceinline__ void mat3_vec_fold(const uint32_t* A, uint32_t* s)
+{
+    uint32_t x[3];
+#pragma unroll
+    for(int r = 0; r < 3; r++)
+    {
+        uint64_t acc = 0;
+#pragma unroll
+        for(int c = 0; c < 3; c++)
+            acc += fold_mod<K, C>((uint64_t)A[3 * r + c] * s[c]);
+        x[r] = fold_mod<K, C>(acc);
+    }
+    s[0] = x[0];
+    s[1] = x[1];
+    s[2] = x[2];
 }
 
 // 3x3 matrix (row-major, entries < m) times state vector, mod m.

@@ -140,13 +140,16 @@ __global__ __launch_bounds__(256) void generate_chunked_kernel(interleaved_call 
     alignas(16) out_t o[OUT];
     uint32_t     in[IN];
     const uint64_t k1 = ending ? steps : (k0 + C < steps ? k0 + C : steps);
-    for(uint64_t k = k0; k < k1; k++)
+    // 32-bit loop state; three steps per iteration turn the state rotation into renaming
+    const uint32_t cnt  = (uint32_t)(k1 - k0);
+    size_t         index = id + k0 * S;
+#pragma unroll 3
+    for(uint32_t k = 0; k < cnt; k++, index += S)
     {
 #pragma unroll
         for(int i = 0; i < IN; i++)
             in[i] = eng.next();
         D::apply(in, o, c.params);
-        const size_t index = id + k * S;
         if constexpr(BYTES == 4)
             *reinterpret_cast<uint32_t*>(vec_data + index * OUT) = *reinterpret_cast<const uint32_t*>(o);
         else if constexpr(BYTES == 8)

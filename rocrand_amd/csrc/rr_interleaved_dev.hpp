@@ -54,8 +54,8 @@ struct ieng<inter_kind::mrg32k3a>
         for(int i = 0; v > 0; v >>= 1, i++)
             if(v & 1)
             {
-                mat3_vec_mod(tab + (i * 2 + 0) * 9, s.g1, MRG32_M1);
-                mat3_vec_mod(tab + (i * 2 + 1) * 9, s.g2, MRG32_M2);
+                mat3_vec_fold<32, 209>(tab + (i * 2 + 0) * 9, s.g1);
+                mat3_vec_fold<32, 22853>(tab + (i * 2 + 1) * 9, s.g2);
             }
     }
     __device__ void init(const device_tables& t, uint32_t e, uint64_t off)
@@ -75,8 +75,8 @@ struct ieng<inter_kind::mrg31k3p> : ieng<inter_kind::mrg32k3a>
         for(int i = 0; v > 0; v >>= 1, i++)
             if(v & 1)
             {
-                mat3_vec_mod(tab + (i * 2 + 0) * 9, s.g1, MRG31_M1);
-                mat3_vec_mod(tab + (i * 2 + 1) * 9, s.g2, MRG31_M2);
+                mat3_vec_fold<31, 1>(tab + (i * 2 + 0) * 9, s.g1);
+                mat3_vec_fold<31, 21069>(tab + (i * 2 + 1) * 9, s.g2);
             }
     }
     __device__ void init(const device_tables& t, uint32_t e, uint64_t off)

@@ -45,7 +45,8 @@ def _mix(count, seed, hi):
     kinds = list(DT)
     for i, s in enumerate(specs):
         s.kind = kinds[i % len(kinds)]
-        if s.kind in (R.BATCH_NORMAL_DOUBLE, R.BATCH_LOG_NORMAL_FLOAT, R.BATCH_LOG_NORMAL_DOUBLE):
+        if s.kind in (R.BATCH_NORMAL_FLOAT, R.BATCH_NORMAL_DOUBLE, R.BATCH_LOG_NORMAL_FLOAT,
+                      R.BATCH_LOG_NORMAL_DOUBLE):
             s.a, s.b = 0.25, 1.5
         if s.kind == R.BATCH_POISSON:
             s.a = [0.5, 4.0, 1000.0, 5000.0][i % 4]
