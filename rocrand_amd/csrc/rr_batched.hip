@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void batch_inter_kernel(const inter_call* call
         const uint32_t    eid = (id + c.start) % S;
         ieng<K>           g;
         g.set(c.base[eid < c.start ? 1 : 0]);
-        g.jump(t.mrg32_sub, eid);
+        g.sub(t, eid);
         out_t* const data     = static_cast<out_t*>(c.data);
         out_t* const vec_data = data + c.mis;
         alignas(16) out_t o[OUT];

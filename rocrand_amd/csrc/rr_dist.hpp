@@ -12,6 +12,7 @@
 #pragma once
 
 #include "rr_engines.hpp"
+#include "rr_mathtab.hpp"
 
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
@@ -36,25 +37,49 @@ constexpr float  R_2PI            = 6.2831855f;
 constexpr float  R_SQRT2          = 1.4142135f;
 
 // ---------------------------------------------------------------- scalar math
-// Float transcendentals evaluated in double (relative error < 2^-50) and rounded once:
-// correctly rounded except where the exact value lies within 2^-50 of a float rounding
-// boundary, so each differs from the host libm's float result by at most the host's own
-// error (< 1 ulp).  Written for the arguments the transforms produce, with short fixed
-// polynomials instead of the general-purpose library routines:
-//   log: x = 2^e m, m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh(f), f = (m-1)/(m+1), |f| < 0.172,
-//        series to f^21;
+// Float transcendentals evaluated in double and rounded once, written for the
+// arguments the transforms produce, with short fixed polynomials instead of the
+// general-purpose library routines:
+//   logf_tab (Box-Muller radius, u a positive normal float): u = 2^k z with
+//        z in [0.699, 1.398), ln u = k ln 2 - ln(1/c_i) + log1p(z/c_i - 1) over the 64-entry
+//        table of rr_mathtab.hpp, |z/c_i - 1| < 2^-7, series to r^6 (relative error < 2^-44);
+//   log (erfinv argument, any float): x = 2^e m, m in [sqrt(1/2), sqrt(2)),
+//        ln m = 2 atanh(f), f = (m-1)/(m+1), |f| < 0.172, series to f^21;
 //   sin/cos for x in [0, 8): Cody-Waite reduction by pi/2 (two parts, fused), Taylor
-//        series to r^15 / r^16 on |r| <= pi/4;
-//   exp: reduction by ln 2 (two parts), Taylor series to r^13 on |r| <= 0.347, scaling by
-//        2^k in double.
+//        series to r^13 / r^14 on |r| <= pi/4;
+//   expf_acc: reduction by ln2/32 (two parts), 2^(j/32) from rr_mathtab.hpp, Taylor series
+//        of exp(r) - 1 to r^5 on |r| <= ln2/64, scaling by 2^e in double.
 // A host C twin of these routines (exact division in place of the refined reciprocal)
-// rounds to the correctly rounded float result for every float in [2^-32, 1] (log), every
-// float in [0, 7] (sin, cos) and a 1/37 sample of all floats (exp).
+// rounds to the correctly rounded float result for every float in [2^-32, 1] (logf_tab
+// and log), every float in (0, 7] but one (sin; that one lies within 2^-48 of a rounding
+// boundary), every float in (0, 7] (cos) and every float in (-105, 90) but two (exp).
+// So each differs from the host libm's float result by at most the host's own error
+// (< 1 ulp).
 constexpr double RR_LN2_HI  = 6.93147180369123816490e-01; // ln 2, top 32 bits
 constexpr double RR_LN2_LO  = 1.90821492927058770002e-10; // ln 2 - RR_LN2_HI
+constexpr double RR_LN2     = 6.93147180559945286227e-01; // ln 2 rounded to double
 constexpr double RR_PIO2_1  = 1.57079632679489655800e+00; // pi/2 rounded to double
 constexpr double RR_PIO2_2  = 6.12323399573676603587e-17; // pi/2 - RR_PIO2_1
 constexpr double RR_2_OV_PI = 6.36619772367581382433e-01;
+
+// ln u for a positive normal float u (the Box-Muller uniforms lie in [2^-32, 1]).
+__device__ __forceinline__ float logf_tab(float u)
+{
+    const uint32_t ix  = __float_as_uint(u);
+    const uint32_t tmp = ix - RR_LOGTAB_OFF;
+    const uint32_t i   = (tmp >> (23 - RR_LOGTAB_BITS)) & ((1u << RR_LOGTAB_BITS) - 1u);
+    const int      k   = (int)tmp >> 23;
+    const float    z   = __uint_as_float(ix - (tmp & 0xff800000u));
+    const double2  t   = rr_logtab[i];
+    const double   r   = __builtin_fma((double)z, t.x, -1.0);
+    const double   r2  = r * r;
+    double         p   = __builtin_fma(r, -1.0 / 6, 1.0 / 5);
+    p                  = __builtin_fma(p, r, -0.25);
+    p                  = __builtin_fma(p, r, 1.0 / 3);
+    p                  = __builtin_fma(p, r, -0.5);
+    const double hi    = __builtin_fma((double)k, RR_LN2, t.y);
+    return (float)(hi + __builtin_fma(p, r2, r));
+}
 
 __device__ __forceinline__ double log_dbl(double x)
 {
@@ -90,73 +115,58 @@ __device__ __forceinline__ double log_dbl(double x)
     return y;
 }
 
-__device__ __forceinline__ void sincos_dbl(double x, double* s, double* c)
+// sin and cos of x in [0, 8), rounded to float; the quadrant fix-up is done on the
+// rounded values (rounding commutes with negation).
+__device__ __forceinline__ void sincosf_acc(float xf, float* s, float* c)
 {
+    const double x  = (double)xf;
     const double q  = __builtin_rint(x * RR_2_OV_PI);
     double       r  = __builtin_fma(-q, RR_PIO2_1, x);
     r               = __builtin_fma(-q, RR_PIO2_2, r);
     const double r2 = r * r;
-    double       ps = -1.0 / 1307674368000.0;
-    ps              = __builtin_fma(ps, r2, 1.0 / 6227020800.0);
+    double       ps = 1.0 / 6227020800.0;
     ps              = __builtin_fma(ps, r2, -1.0 / 39916800.0);
     ps              = __builtin_fma(ps, r2, 1.0 / 362880.0);
     ps              = __builtin_fma(ps, r2, -1.0 / 5040.0);
     ps              = __builtin_fma(ps, r2, 1.0 / 120.0);
     ps              = __builtin_fma(ps, r2, -1.0 / 6.0);
-    const double sn = __builtin_fma(r * r2, ps, r);
-    double       pc = 1.0 / 20922789888000.0;
-    pc              = __builtin_fma(pc, r2, -1.0 / 87178291200.0);
+    const float  sn = (float)__builtin_fma(r * r2, ps, r);
+    double       pc = -1.0 / 87178291200.0;
     pc              = __builtin_fma(pc, r2, 1.0 / 479001600.0);
     pc              = __builtin_fma(pc, r2, -1.0 / 3628800.0);
     pc              = __builtin_fma(pc, r2, 1.0 / 40320.0);
     pc              = __builtin_fma(pc, r2, -1.0 / 720.0);
     pc              = __builtin_fma(pc, r2, 1.0 / 24.0);
     pc              = __builtin_fma(pc, r2, -0.5);
-    const double cs = __builtin_fma(r2, pc, 1.0);
+    const float  cs = (float)__builtin_fma(r2, pc, 1.0);
     const int    qi = (int)q & 3;
-    const double a  = (qi & 1) ? cs : sn;
-    const double b  = (qi & 1) ? sn : cs;
+    const float  a  = (qi & 1) ? cs : sn;
+    const float  b  = (qi & 1) ? sn : cs;
     *s              = (qi & 2) ? -a : a;
     *c              = ((qi + 1) & 2) ? -b : b;
 }
 
-__device__ __forceinline__ double exp_dbl(double x)
-{
-    const double xc = x < -110.0 ? -110.0 : (x > 100.0 ? 100.0 : x);
-    const double k  = __builtin_rint(xc * 1.44269504088896338700);
-    double       r  = __builtin_fma(-k, RR_LN2_HI, xc);
-    r               = __builtin_fma(-k, RR_LN2_LO, r);
-    double p = 1.0 / 6227020800.0;
-    p        = __builtin_fma(p, r, 1.0 / 479001600.0);
-    p        = __builtin_fma(p, r, 1.0 / 39916800.0);
-    p        = __builtin_fma(p, r, 1.0 / 3628800.0);
-    p        = __builtin_fma(p, r, 1.0 / 362880.0);
-    p        = __builtin_fma(p, r, 1.0 / 40320.0);
-    p        = __builtin_fma(p, r, 1.0 / 5040.0);
-    p        = __builtin_fma(p, r, 1.0 / 720.0);
-    p        = __builtin_fma(p, r, 1.0 / 120.0);
-    p        = __builtin_fma(p, r, 1.0 / 24.0);
-    p        = __builtin_fma(p, r, 1.0 / 6.0);
-    p        = __builtin_fma(p, r, 0.5);
-    p        = __builtin_fma(p, r, 1.0);
-    p        = __builtin_fma(p, r, 1.0);
-    double y = __builtin_amdgcn_ldexp(p, (int)k);
-    if(x < -110.0)
-        y = 0.0;
-    if(x > 100.0)
-        y = __builtin_inf();
-    return x != x ? x : y;
-}
-
 __device__ __forceinline__ float logf_acc(float x) { return (float)log_dbl((double)x); }
-__device__ __forceinline__ float expf_acc(float x) { return (float)exp_dbl((double)x); }
-// x in [0, 8): the Box-Muller angles
-__device__ __forceinline__ void sincosf_acc(float x, float* s, float* c)
+
+// exp of a float, rounded to float: x = (32 e + j) ln2/32 + r, exp x = 2^e 2^(j/32) exp(r)
+// with the 2^(j/32) table of rr_mathtab.hpp.  Arguments are clamped to [-110, 100]
+// first (the float result is then 0 or +inf already), which keeps |32 e + j| < 2^13.
+__device__ __forceinline__ float expf_acc(float xf)
 {
-    double sd, cd;
-    sincos_dbl((double)x, &sd, &cd);
-    *s = (float)sd;
-    *c = (float)cd;
+    const double x  = (double)xf;
+    const double xc = x < -110.0 ? -110.0 : (x > 100.0 ? 100.0 : x);
+    const double kd = __builtin_rint(xc * RR_EXP_INV_STEP);
+    const int    k  = (int)kd;
+    double       r  = __builtin_fma(-kd, RR_EXP_STEP_HI, xc);
+    r               = __builtin_fma(-kd, RR_EXP_STEP_LO, r);
+    double p        = __builtin_fma(r, 1.0 / 120, 1.0 / 24);
+    p               = __builtin_fma(p, r, 1.0 / 6);
+    p               = __builtin_fma(p, r, 0.5);
+    p               = __builtin_fma(p, r, 1.0);
+    p               = p * r;
+    const double t  = rr_exptab[k & ((1 << RR_EXPTAB_BITS) - 1)];
+    const float  y  = (float)__builtin_amdgcn_ldexp(__builtin_fma(t, p, t), k >> RR_EXPTAB_BITS);
+    return xf != xf ? xf : y;
 }
 
 // rocrand_uniform.h:63-66: 2^-32 + v * 2^-32.  The product by a power of two is
@@ -224,7 +234,7 @@ __device__ __forceinline__ void box_muller(uint32_t x, uint32_t y, float& r0, fl
 {
     const float u = uniform_f32(x);
     const float v = __fadd_rn(R_2POW32_INV_2PI, __fmul_rn((float)y, R_2POW32_INV_2PI));
-    const float s = sqrtf(__fmul_rn(-2.0f, logf_acc(u)));
+    const float s = sqrtf(__fmul_rn(-2.0f, logf_tab(u)));
     float       sn, cs;
     sincosf_acc(v, &sn, &cs);
     r0 = __fmul_rn(sn, s);
@@ -250,7 +260,7 @@ __device__ __forceinline__ void box_muller_half(uint32_t x16, uint32_t y16, floa
 {
     const float u = __fadd_rn(R_2POW16_INV, __fmul_rn((float)(x16 & 0xffffu), R_2POW16_INV));
     const float v = __fadd_rn(R_2POW16_INV_2PI, __fmul_rn((float)(y16 & 0xffffu), R_2POW16_INV_2PI));
-    const float s = sqrtf(__fmul_rn(-2.0f, logf_acc(u)));
+    const float s = sqrtf(__fmul_rn(-2.0f, logf_tab(u)));
     float       sn, cs;
     sincosf_acc(v, &sn, &cs);
     h0 = __half2float(__float2half(__fmul_rn(sn, s)));
@@ -262,7 +272,7 @@ __device__ __forceinline__ void mrg_box_muller(uint32_t x, uint32_t y, float& r0
 {
     const float u = mrg_f32<MRG>(x);
     const float v = __fmul_rn(mrg_f32<MRG>(y), R_2PI);
-    const float s = sqrtf(__fmul_rn(-2.0f, logf_acc(u)));
+    const float s = sqrtf(__fmul_rn(-2.0f, logf_tab(u)));
     float       sn, cs;
     sincosf_acc(v, &sn, &cs);
     r0 = __fmul_rn(sn, s);

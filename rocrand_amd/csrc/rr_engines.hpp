@@ -335,10 +335,16 @@ __host__ __device__ __forceinline__ uint32_t mrg32_reduce_m2(uint64_t p)
 
 // x mod m for x < 2m + 2^32 and m = 2^32 - c, from the 64-bit value's words: x - m is
 // lo + c (mod 2^32) exactly when x >= m, i.e. when hi != 0 or lo >= m.
+// x mod m for x < 2m, m = 2^32 - c, in three 32-bit operations and no lane masks:
+// v = x mod 2^32 + [x >= 2^32] c is x - m when x >= 2^32 (then v < m) and x otherwise;
+// v + c wraps to v - m exactly when v >= m, so the smaller of v and v + c (mod 2^32)
+// is the residue.
 __host__ __device__ __forceinline__ uint32_t mrg32_fold_final(uint64_t x, uint32_t c)
 {
     const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-    return lo + ((hi != 0u || lo >= 0u - c) ? c : 0u);
+    const uint32_t v  = lo + ((0u - hi) & c);
+    const uint32_t w  = v + c;
+    return w < v ? w : v;
 }
 
 __host__ __device__ __forceinline__ uint32_t mrg32k3a_next(mrg_state& s)
@@ -361,23 +367,30 @@ __host__ __device__ __forceinline__ uint32_t mrg32k3a_next(mrg_state& s)
     return (p1 - p2) + (p1 <= p2 ? (uint32_t)MRG32_M1 : 0u);
 }
 
+// x - m if x >= m, else x (x < 2^32): the smaller of x and x - m (mod 2^32).
+__host__ __device__ __forceinline__ uint32_t sub_if_ge(uint32_t x, uint32_t m)
+{
+    const uint32_t y = x - m;
+    return y < x ? y : x;
+}
+
 __host__ __device__ __forceinline__ uint32_t mrg31k3p_next(mrg_state& s)
 {
     uint32_t tmp = (((s.g1[1] & 511u) << 22) + (s.g1[1] >> 9)) + (((s.g1[2] & 16777215u) << 7) + (s.g1[2] >> 24));
-    tmp -= (tmp >= MRG31_M1) ? MRG31_M1 : 0;
+    tmp          = sub_if_ge(tmp, MRG31_M1);
     tmp += s.g1[2];
-    tmp -= (tmp >= MRG31_M1) ? MRG31_M1 : 0;
+    tmp     = sub_if_ge(tmp, MRG31_M1);
     s.g1[2] = s.g1[1];
     s.g1[1] = s.g1[0];
     s.g1[0] = tmp;
     tmp     = (((s.g2[0] & 65535u) << 15) + 21069u * (s.g2[0] >> 16));
-    tmp -= (tmp >= MRG31_M2) ? MRG31_M2 : 0;
+    tmp     = sub_if_ge(tmp, MRG31_M2);
     tmp += ((s.g2[2] & 65535u) << 15);
-    tmp -= (tmp >= MRG31_M2) ? MRG31_M2 : 0;
+    tmp = sub_if_ge(tmp, MRG31_M2);
     tmp += 21069u * (s.g2[2] >> 16);
-    tmp -= (tmp >= MRG31_M2) ? MRG31_M2 : 0;
+    tmp = sub_if_ge(tmp, MRG31_M2);
     tmp += s.g2[2];
-    tmp -= (tmp >= MRG31_M2) ? MRG31_M2 : 0;
+    tmp = sub_if_ge(tmp, MRG31_M2);
     s.g2[2] = s.g2[1];
     s.g2[1] = s.g2[0];
     s.g2[0] = tmp;
@@ -397,7 +410,18 @@ __host__ __device__ __forceinline__ uint32_t fold_mod(uint64_t x)
     return (uint32_t)(x >= m ? x - m : x);
 }
 
-// 3x3 matrix (row-major, entries < m) times state vector, mod m = 2^K - C.
+// One fold by 2^K = C (mod 2^K - C): a congruent value below 2^48 for the operands of
+// mat3_vec_fold.
+template<uint32_t K, uint32_t C>
+__host__ __device__ __forceinline__ uint64_t fold_once(uint64_t x)
+{
+    return (x >> K) * C + (x & ((1ull << K) - 1));
+}
+
+// 3x3 matrix (row-major, entries < m) times state vector (any 32-bit words), mod
+// m = 2^K - C.  Each row accumulates with one fold between products: an entry below m
+// times a 32-bit word plus a folded partial stays below 2^64 for the four moduli used
+// here, and fold_mod returns the canonical residue of the last sum.
 template<uint32_t K, uint32_t C>
 __host__ __device__ __forceinline__ void mat3_vec_fold(const uint32_t* A, uint32_t* s)
 {
@@ -405,11 +429,9 @@ __host__ __device__ __forceinline__ void mat3_vec_fold(const uint32_t* A, uint32
 #pragma unroll
     for(int r = 0; r < 3; r++)
     {
-        uint64_t acc = 0;
-#pragma unroll
-        for(int c = 0; c < 3; c++)
-            acc += fold_mod<K, C>((uint64_t)A[3 * r + c] * s[c]);
-        x[r] = fold_mod<K, C>(acc);
+        uint64_t acc = fold_once<K, C>((uint64_t)A[3 * r + 0] * s[0]);
+        acc          = fold_once<K, C>((uint64_t)A[3 * r + 1] * s[1] + acc);
+        x[r]         = fold_mod<K, C>((uint64_t)A[3 * r + 2] * s[2] + acc);
     }
     s[0] = x[0];
     s[1] = x[1];

@@ -16,6 +16,39 @@ constexpr uint32_t S = INTERLEAVED_ENGINES;
 template<inter_kind K>
 struct ieng;
 
+// One radix-table entry (both components) applied to the state.
+template<uint32_t K, uint32_t C1, uint32_t C2>
+__device__ __forceinline__ void mrg_entry_apply(const uint32_t* entry, mrg_state& s)
+{
+    const uint4* p = reinterpret_cast<const uint4*>(entry);
+    uint32_t     w[MRG_RADIX_WORDS];
+#pragma unroll
+    for(int i = 0; i < MRG_RADIX_WORDS / 4; i++)
+    {
+        const uint4 q = p[i];
+        w[4 * i + 0]  = q.x;
+        w[4 * i + 1]  = q.y;
+        w[4 * i + 2]  = q.z;
+        w[4 * i + 3]  = q.w;
+    }
+    mat3_vec_fold<K, C1>(w, s.g1);
+    mat3_vec_fold<K, C2>(w + 9, s.g2);
+}
+
+// Jump to subsequence e (e < 2^17) with the radix tables: X^(e mod 512), then
+// X^(512 (e / 512)); the powers commute.  Same state as the binary expansion over the
+// A^(2^(sub + i)) table of the reference's skipahead_subsequence.
+template<uint32_t K, uint32_t C1, uint32_t C2>
+__device__ __forceinline__ void mrg_sub_jump(const uint32_t* tab, uint32_t e, mrg_state& s)
+{
+    const uint32_t lo = e & (uint32_t)(MRG_RADIX_LO - 1);
+    const uint32_t hi = e >> MRG_RADIX_LO_BITS;
+    if(lo != 0u)
+        mrg_entry_apply<K, C1, C2>(tab + lo * MRG_RADIX_WORDS, s);
+    if(hi != 0u)
+        mrg_entry_apply<K, C1, C2>(tab + (MRG_RADIX_LO + hi) * MRG_RADIX_WORDS, s);
+}
+
 template<>
 struct ieng<inter_kind::mrg32k3a>
 {
@@ -58,9 +91,10 @@ struct ieng<inter_kind::mrg32k3a>
                 mat3_vec_fold<32, 22853>(tab + (i * 2 + 1) * 9, s.g2);
             }
     }
+    __device__ void sub(const device_tables& t, uint32_t e) { mrg_sub_jump<32, 209, 22853>(t.mrg32_subr, e, s); }
     __device__ void init(const device_tables& t, uint32_t e, uint64_t off)
     {
-        jump(t.mrg32_sub, e);
+        sub(t, e);
         jump(t.mrg32_step, off);
     }
 };
@@ -79,9 +113,10 @@ struct ieng<inter_kind::mrg31k3p> : ieng<inter_kind::mrg32k3a>
                 mat3_vec_fold<31, 21069>(tab + (i * 2 + 1) * 9, s.g2);
             }
     }
+    __device__ void sub(const device_tables& t, uint32_t e) { mrg_sub_jump<31, 1, 21069>(t.mrg31_subr, e, s); }
     __device__ void init(const device_tables& t, uint32_t e, uint64_t off)
     {
-        jump(t.mrg31_sub, e);
+        sub(t, e);
         jump(t.mrg31_step, off);
     }
 };

@@ -99,6 +99,13 @@ struct sobol_device_tables
 };
 hipError_t get_sobol_device_tables(sobol_device_tables& t);
 
+// Subsequence radix tables: digit 0 = e mod 512, digit 1 = e / 512 (e < 2^17).
+constexpr int MRG_RADIX_LO_BITS = 9;
+constexpr int MRG_RADIX_LO      = 1 << MRG_RADIX_LO_BITS;
+constexpr int MRG_RADIX_HI      = 256;
+constexpr int MRG_RADIX_WORDS   = 20;
+static_assert((uint64_t)MRG_RADIX_LO * MRG_RADIX_HI >= INTERLEAVED_ENGINES, "radix tables cover every engine");
+
 // Host-computed jump tables, uploaded once per device (rr_tables.cpp).
 struct device_tables
 {
@@ -106,6 +113,8 @@ struct device_tables
     const uint32_t* mrg32_sub;  // [64][2][9]  A^(2^(76+i))
     const uint32_t* mrg31_step;
     const uint32_t* mrg31_sub;  // A^(2^(72+i))
+    const uint32_t* mrg32_subr; // [512 + 256][20]  A^(d 2^76), A^(512 d 2^76)
+    const uint32_t* mrg31_subr; // [512 + 256][20]  A^(d 2^72), A^(512 d 2^72)
     const uint32_t* xorwow_jump; // [32][800]  M^(4^k)
     const uint32_t* xorwow_seq;  // [32][800]  M^(2^67 * 4^k)
     const uint32_t* lfsr_jump;   // [32][512]

@@ -7,6 +7,7 @@
 // They are pure matrix powers, so they are derived here instead of carried:
 //   MRG32k3a: A^(2^i) and A^(2^(76+i))       (subsequence = 2^76 steps)
 //   MRG31k3p: A^(2^i) and A^(2^(72+i))       (subsequence = 2^72 steps)
+//   both    : radix-512/256 subsequence tables (engine e < 2^17 in two products)
 //   XORWOW  : M^(4^k) and M^(2^67 * 4^k)     (GF(2), 160x160)
 //   LFSR113 : M^(4^k) and M^(2^55 * 4^k)     (GF(2), 128x128)
 #include "rr_launch.hpp"
@@ -55,6 +56,31 @@ void mrg_table(const uint64_t A1[9], const uint64_t A2[9], uint64_t m1, uint64_t
             mat3_mul(X, X, X, m[c]);
         }
     }
+}
+
+// Radix tables of the subsequence jump: entry d (d < 512) holds X^d and entry 512 + d
+// (d < 256) holds X^(512 d), X = A^(2^first) per component, so an engine index below
+// 2^17 costs two table products.  Entry layout: component 0 (9 words), component 1
+// (9 words), 2 words of padding (16-byte aligned entries).
+void mrg_radix_table(const uint32_t* pow2, uint64_t m1, uint64_t m2, uint32_t* out)
+{
+    const uint64_t m[2] = {m1, m2};
+    for(int c = 0; c < 2; c++)
+        for(int level = 0; level < 2; level++)
+        {
+            uint64_t X[9], P[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+            const int i = level == 0 ? 0 : MRG_RADIX_LO_BITS; // X^(2^i) from the power table
+            for(int j = 0; j < 9; j++)
+                X[j] = pow2[(i * 2 + c) * 9 + j];
+            const int first = level == 0 ? 0 : MRG_RADIX_LO;
+            const int count = level == 0 ? MRG_RADIX_LO : MRG_RADIX_HI;
+            for(int d = 0; d < count; d++)
+            {
+                for(int j = 0; j < 9; j++)
+                    out[(first + d) * MRG_RADIX_WORDS + c * 9 + j] = (uint32_t)P[j];
+                mat3_mul(P, X, P, m[c]);
+            }
+        }
 }
 
 // ---- GF(2) matrices: row (word i, bit j) holds the N-word image of that input bit.
@@ -123,7 +149,7 @@ void lfsr_step(uint32_t* z) { lfsr113_step(z); }
 
 struct host_tables
 {
-    std::vector<uint32_t> mrg32_step, mrg32_sub, mrg31_step, mrg31_sub;
+    std::vector<uint32_t> mrg32_step, mrg32_sub, mrg31_step, mrg31_sub, mrg32_subr, mrg31_subr;
     std::vector<uint32_t> xorwow_jump, xorwow_seq, lfsr_jump, lfsr_seq;
     host_tables()
     {
@@ -145,6 +171,10 @@ struct host_tables
             mrg_table(A1, A2, MRG31_M1, MRG31_M2, 0, mrg31_step.data());
             mrg_table(A1, A2, MRG31_M1, MRG31_M2, 72, mrg31_sub.data());
         }
+        mrg32_subr.assign((MRG_RADIX_LO + MRG_RADIX_HI) * MRG_RADIX_WORDS, 0u);
+        mrg31_subr.assign((MRG_RADIX_LO + MRG_RADIX_HI) * MRG_RADIX_WORDS, 0u);
+        mrg_radix_table(mrg32_sub.data(), MRG32_M1, MRG32_M2, mrg32_subr.data());
+        mrg_radix_table(mrg31_sub.data(), MRG31_M1, MRG31_M2, mrg31_subr.data());
         xorwow_jump.resize(32 * 800);
         xorwow_seq.resize(32 * 800);
         gf2_tables<5>(xorwow_x_step, 67, xorwow_jump.data(), xorwow_seq.data());
@@ -203,6 +233,8 @@ hipError_t get_device_tables(device_tables& t)
         if((err = upload(h.mrg32_sub, &d.mrg32_sub)) != hipSuccess) return err;
         if((err = upload(h.mrg31_step, &d.mrg31_step)) != hipSuccess) return err;
         if((err = upload(h.mrg31_sub, &d.mrg31_sub)) != hipSuccess) return err;
+        if((err = upload(h.mrg32_subr, &d.mrg32_subr)) != hipSuccess) return err;
+        if((err = upload(h.mrg31_subr, &d.mrg31_subr)) != hipSuccess) return err;
         if((err = upload(h.xorwow_jump, &d.xorwow_jump)) != hipSuccess) return err;
         if((err = upload(h.xorwow_seq, &d.xorwow_seq)) != hipSuccess) return err;
         if((err = upload(h.lfsr_jump, &d.lfsr_jump)) != hipSuccess) return err;
