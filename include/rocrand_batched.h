@@ -68,7 +68,11 @@ ROCRANDAPI rocrand_status rocrand_batch_plan_launch(rocrand_batch_plan plan);
 /* Releases the plan once its last launch has finished (work queued later on the stream
  * is not waited for). */
 ROCRANDAPI rocrand_status rocrand_batch_plan_destroy(rocrand_batch_plan plan);
-/* One-shot: plan, launch without a graph, release. */
+/* One-shot: validates and lays out the batch and launches it without a graph,
+ * asynchronously on `stream` (like any rocrand_generate_* call).  The descriptors go
+ * through pooled pinned and device memory that is recycled once the launch has
+ * completed, so repeated calls neither allocate in steady state nor wait for the device;
+ * `streams` may be reused as soon as the call returns. */
 ROCRANDAPI rocrand_status rocrand_generate_batched(const rocrand_batch_stream* streams,
                                                    size_t                      count,
                                                    hipStream_t                 stream);

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -219,21 +220,196 @@ launcher pick_launcher(bool philox, out_kind k)
 
 struct group
 {
-    bool                    philox;
-    out_kind                kind;
-    launcher                fn;
+    bool                      philox;
+    out_kind                  kind;
+    launcher                  fn;
     std::vector<counter_call> ccalls;
     std::vector<inter_call>   icalls;
     std::vector<batch_tile>   tiles;
-    size_t                  bytes_out = 0;
-    void*                   d_calls   = nullptr;
-    batch_tile*             d_tiles   = nullptr;
-    uint32_t                blocks    = 2048;
+    size_t                    bytes_out = 0;
+    size_t                    calls_at  = 0; // offsets in the descriptor image
+    size_t                    tiles_at  = 0;
+    void*                     d_calls   = nullptr;
+    batch_tile*               d_tiles   = nullptr;
+    uint32_t                  blocks    = 2048;
 };
 
 // Grid sizes chosen per (engine, kind, log2 of the group's output bytes).
-std::mutex                        g_tune_mtx;
+std::mutex                                    g_tune_mtx;
 std::map<std::tuple<int, int, int>, uint32_t> g_tuned;
+
+// A batch on the host: groups, the distinct Poisson tables, and the layout of the one
+// descriptor image (tables, then each group's calls and tiles) that goes to the device.
+struct batch_layout
+{
+    std::vector<group>                     groups;
+    std::vector<std::vector<double>>       prob;
+    std::vector<std::vector<unsigned int>> alias;
+    std::vector<size_t>                    prob_at, alias_at;
+    size_t                                 bytes = 0;
+
+    static size_t align(size_t x) { return (x + 255) / 256 * 256; }
+
+    void place()
+    {
+        size_t at = 0;
+        for(size_t i = 0; i < prob.size(); i++)
+        {
+            prob_at.push_back(at);
+            at = align(at + prob[i].size() * sizeof(double));
+            alias_at.push_back(at);
+            at = align(at + alias[i].size() * sizeof(unsigned int));
+        }
+        for(auto& g : groups)
+        {
+            g.calls_at = at;
+            at = align(at + (g.philox ? g.ccalls.size() * sizeof(counter_call) : g.icalls.size() * sizeof(inter_call)));
+            g.tiles_at = at;
+            at         = align(at + g.tiles.size() * sizeof(batch_tile));
+        }
+        bytes = std::max<size_t>(at, 256);
+    }
+
+    // Writes the image for device address `dev` into `host` (bytes long): Poisson table
+    // references (held as table indices until now) become device pointers.
+    void pack(char* host, char* dev)
+    {
+        for(size_t i = 0; i < prob.size(); i++)
+        {
+            memcpy(host + prob_at[i], prob[i].data(), prob[i].size() * sizeof(double));
+            memcpy(host + alias_at[i], alias[i].data(), alias[i].size() * sizeof(unsigned int));
+        }
+        auto fix = [&](dist_params& prm) {
+            if(prm.pt.size == 0)
+                return;
+            const size_t t = reinterpret_cast<uintptr_t>(prm.pt.prob);
+            prm.pt.prob    = reinterpret_cast<const double*>(dev + prob_at[t]);
+            prm.pt.alias   = reinterpret_cast<const unsigned int*>(dev + alias_at[t]);
+        };
+        for(auto& g : groups)
+        {
+            if(g.philox)
+            {
+                for(auto& c : g.ccalls)
+                    fix(c.params);
+                memcpy(host + g.calls_at, g.ccalls.data(), g.ccalls.size() * sizeof(counter_call));
+            }
+            else
+            {
+                for(auto& c : g.icalls)
+                    fix(c.params);
+                memcpy(host + g.calls_at, g.icalls.data(), g.icalls.size() * sizeof(inter_call));
+            }
+            memcpy(host + g.tiles_at, g.tiles.data(), g.tiles.size() * sizeof(batch_tile));
+            g.d_calls = dev + g.calls_at;
+            g.d_tiles = reinterpret_cast<batch_tile*>(dev + g.tiles_at);
+        }
+    }
+};
+
+hipError_t run_groups(const std::vector<group>& groups, const device_tables& tables, hipStream_t s)
+{
+    for(const auto& g : groups)
+    {
+        if(g.tiles.empty())
+            continue;
+        const uint32_t blocks = std::min<uint32_t>(g.blocks, (uint32_t)g.tiles.size());
+        hipError_t     e      = g.fn(g.d_calls, g.d_tiles, (uint32_t)g.tiles.size(), blocks, tables, s);
+        if(e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+
+// Device blocks with pinned host twins for the one-shot call's descriptor images,
+// recycled once the call that used them has completed (its event has fired).  Blocks
+// come in power-of-two sizes from 64 KiB and stay allocated for the process lifetime,
+// so a steady stream of calls allocates nothing and never waits for the device.
+class staging_pool
+{
+public:
+    struct block
+    {
+        char*  dev  = nullptr;
+        char*  host = nullptr;
+        size_t size = 0;
+        int    device = 0;
+    };
+
+    hipError_t acquire(int device, size_t bytes, block& b, hipEvent_t& ev)
+    {
+        size_t size = 64 << 10;
+        while(size < bytes)
+            size *= 2;
+        std::lock_guard<std::mutex> lock(mu_);
+        reclaim();
+        auto& fl = free_[std::make_pair(device, size)];
+        if(!fl.empty())
+        {
+            b = fl.back();
+            fl.pop_back();
+        }
+        else
+        {
+            b        = block{};
+            b.size   = size;
+            b.device = device;
+            hipError_t e = hipMalloc(reinterpret_cast<void**>(&b.dev), size);
+            if(e != hipSuccess)
+                return e;
+            e = hipHostMalloc(reinterpret_cast<void**>(&b.host), size, hipHostMallocDefault);
+            if(e != hipSuccess)
+            {
+                (void)hipFree(b.dev);
+                return e;
+            }
+        }
+        auto& evs = events_[device];
+        if(!evs.empty())
+        {
+            ev = evs.back();
+            evs.pop_back();
+            return hipSuccess;
+        }
+        return hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    }
+
+    // Hands the block back once `ev` (recorded after the block's last use) has fired.
+    void release_after(const block& b, hipEvent_t ev)
+    {
+        std::lock_guard<std::mutex> lock(mu_);
+        pending_.push_back(std::make_pair(b, ev));
+    }
+
+private:
+    void reclaim()
+    {
+        size_t keep = 0;
+        for(size_t i = 0; i < pending_.size(); i++)
+        {
+            if(hipEventQuery(pending_[i].second) == hipSuccess)
+            {
+                const block& b = pending_[i].first;
+                free_[std::make_pair(b.device, b.size)].push_back(b);
+                events_[b.device].push_back(pending_[i].second);
+            }
+            else
+                pending_[keep++] = pending_[i];
+        }
+        pending_.resize(keep);
+    }
+
+    std::mutex                                        mu_;
+    std::map<std::pair<int, size_t>, std::vector<block>> free_;
+    std::map<int, std::vector<hipEvent_t>>            events_;
+    std::vector<std::pair<block, hipEvent_t>>         pending_;
+};
+
+staging_pool& pool()
+{
+    static staging_pool* p = new staging_pool(); // lives to process exit
+    return *p;
+}
 
 } // namespace
 } // namespace rr
@@ -242,15 +418,14 @@ using namespace rr;
 
 struct rocrand_batch_plan_st
 {
-    hipStream_t                stream = 0;
-    std::vector<group>         groups;
-    std::vector<double*>       d_prob;
-    std::vector<unsigned int*> d_alias;
-    device_tables              tables{};
-    hipGraph_t                 graph = nullptr;
-    hipGraphExec_t             exec  = nullptr;
-    hipEvent_t                 done  = nullptr; // recorded after each launch
-    bool                       launched = false;
+    hipStream_t        stream = 0;
+    std::vector<group> groups;
+    char*              d_image = nullptr;
+    device_tables      tables{};
+    hipGraph_t         graph    = nullptr;
+    hipGraphExec_t     exec     = nullptr;
+    hipEvent_t         done     = nullptr; // recorded after each launch
+    bool               launched = false;
 
     ~rocrand_batch_plan_st()
     {
@@ -260,39 +435,19 @@ struct rocrand_batch_plan_st
             (void)hipGraphExecDestroy(exec);
         if(graph)
             (void)hipGraphDestroy(graph);
-        for(auto& g : groups)
-        {
-            (void)hipFree(g.d_calls);
-            (void)hipFree(g.d_tiles);
-        }
-        for(auto* p : d_prob)
-            (void)hipFree(p);
-        for(auto* p : d_alias)
-            (void)hipFree(p);
-    }
-
-    hipError_t run_groups(hipStream_t s) const
-    {
-        for(const auto& g : groups)
-        {
-            if(g.tiles.empty())
-                continue;
-            const uint32_t blocks = std::min<uint32_t>(g.blocks, (uint32_t)g.tiles.size());
-            hipError_t     e      = g.fn(g.d_calls, g.d_tiles, (uint32_t)g.tiles.size(), blocks, tables, s);
-            if(e != hipSuccess)
-                return e;
-        }
-        return hipSuccess;
+        if(d_image)
+            (void)hipFree(d_image);
     }
 };
 
 namespace
 {
 
-rocrand_status build(rocrand_batch_plan_st& p, const rocrand_batch_stream* streams, size_t count)
+// Validates the streams and lays out the batch on the host (no device calls).
+rocrand_status build(batch_layout& L, const rocrand_batch_stream* streams, size_t count)
 {
     // Poisson alias tables, one per distinct lambda (poisson.hpp:110-161, discrete.hpp:189-265)
-    std::map<double, alias_table> tables;
+    std::map<double, alias_table>         tables;
     std::map<std::pair<int, int>, size_t> gid; // (philox, kind) -> group index
     for(size_t i = 0; i < count; i++)
     {
@@ -326,24 +481,14 @@ rocrand_status build(rocrand_batch_plan_st& p, const rocrand_batch_stream* strea
                     std::vector<unsigned int> alias;
                     unsigned int              off  = 0;
                     const unsigned int        size = poisson_table(b.a, off, prob, alias);
-                    double*                   dp   = nullptr;
-                    unsigned int*             da   = nullptr;
-                    if(hipMalloc(&dp, size * sizeof(double)) != hipSuccess)
-                        return ROCRAND_STATUS_ALLOCATION_FAILED;
-                    p.d_prob.push_back(dp);
-                    if(hipMalloc(&da, size * sizeof(unsigned int)) != hipSuccess)
-                        return ROCRAND_STATUS_ALLOCATION_FAILED;
-                    p.d_alias.push_back(da);
-                    if(hipMemcpy(dp, prob.data(), size * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
-                       || hipMemcpy(da, alias.data(), size * sizeof(unsigned int), hipMemcpyHostToDevice)
-                              != hipSuccess)
-                        return ROCRAND_STATUS_INTERNAL_ERROR;
-                    alias_table t{};
+                    alias_table               t{};
                     t.size   = size;
                     t.offset = off;
-                    t.prob   = dp;
-                    t.alias  = da;
-                    it       = tables.emplace(b.a, t).first;
+                    // the table's index until the image is packed
+                    t.prob = reinterpret_cast<const double*>(static_cast<uintptr_t>(L.prob.size()));
+                    L.prob.push_back(std::move(prob));
+                    L.alias.push_back(std::move(alias));
+                    it = tables.emplace(b.a, t).first;
                 }
                 prm.pt = it->second;
             }
@@ -358,10 +503,10 @@ rocrand_status build(rocrand_batch_plan_st& p, const rocrand_batch_stream* strea
             g.fn     = pick_launcher(philox, k);
             if(!g.fn)
                 return ROCRAND_STATUS_TYPE_ERROR;
-            p.groups.push_back(std::move(g));
-            git = gid.emplace(key, p.groups.size() - 1).first;
+            L.groups.push_back(std::move(g));
+            git = gid.emplace(key, L.groups.size() - 1).first;
         }
-        group&         g  = p.groups[git->second];
+        group&         g  = L.groups[git->second];
         const uint32_t si = (uint32_t)(philox ? g.ccalls.size() : g.icalls.size());
         dist_shape     s{};
         if(philox)
@@ -420,21 +565,14 @@ rocrand_status build(rocrand_batch_plan_st& p, const rocrand_batch_stream* strea
         }
         g.bytes_out += b.n * (size_t)s.esz;
     }
-    for(auto& g : p.groups)
-    {
-        const size_t cb = g.philox ? g.ccalls.size() * sizeof(counter_call) : g.icalls.size() * sizeof(inter_call);
-        const void*  src = g.philox ? (const void*)g.ccalls.data() : (const void*)g.icalls.data();
-        if(hipMalloc(&g.d_calls, std::max<size_t>(cb, 16)) != hipSuccess
-           || hipMalloc(&g.d_tiles, std::max<size_t>(g.tiles.size() * sizeof(batch_tile), 16)) != hipSuccess)
-            return ROCRAND_STATUS_ALLOCATION_FAILED;
-        if(hipMemcpy(g.d_calls, src, cb, hipMemcpyHostToDevice) != hipSuccess
-           || hipMemcpy(g.d_tiles, g.tiles.data(), g.tiles.size() * sizeof(batch_tile), hipMemcpyHostToDevice)
-                  != hipSuccess)
-            return ROCRAND_STATUS_INTERNAL_ERROR;
-    }
-    if(get_device_tables(p.tables) != hipSuccess)
-        return ROCRAND_STATUS_INTERNAL_ERROR;
+    L.place();
     return ROCRAND_STATUS_SUCCESS;
+}
+
+std::tuple<int, int, int> tune_key(const group& g)
+{
+    const int lg = g.bytes_out ? (int)std::log2((double)g.bytes_out) : 0;
+    return std::make_tuple(g.philox ? 1 : 0, (int)g.kind, lg);
 }
 
 // Chooses each group's grid size: the cached choice for its shape, else the fastest of a
@@ -447,8 +585,7 @@ rocrand_status tune(rocrand_batch_plan_st& p)
     {
         if(g.tiles.empty())
             continue;
-        const int  lg  = g.bytes_out ? (int)std::log2((double)g.bytes_out) : 0;
-        const auto key = std::make_tuple(g.philox ? 1 : 0, (int)g.kind, lg);
+        const auto key = tune_key(g);
         {
             std::lock_guard<std::mutex> lock(g_tune_mtx);
             auto                        it = g_tuned.find(key);
@@ -512,7 +649,23 @@ rocrand_status ROCRANDAPI rocrand_batch_plan_create(rocrand_batch_plan*         
     {
         p         = new rocrand_batch_plan_st();
         p->stream = stream;
-        rocrand_status st = build(*p, streams, count);
+        batch_layout   L;
+        rocrand_status st = build(L, streams, count);
+        if(st == ROCRAND_STATUS_SUCCESS)
+        {
+            std::vector<char> image(L.bytes);
+            if(hipMalloc(reinterpret_cast<void**>(&p->d_image), L.bytes) != hipSuccess)
+                st = ROCRAND_STATUS_ALLOCATION_FAILED;
+            else
+            {
+                L.pack(image.data(), p->d_image);
+                if(hipMemcpy(p->d_image, image.data(), L.bytes, hipMemcpyHostToDevice) != hipSuccess)
+                    st = ROCRAND_STATUS_INTERNAL_ERROR;
+            }
+            p->groups = std::move(L.groups);
+        }
+        if(st == ROCRAND_STATUS_SUCCESS && get_device_tables(p->tables) != hipSuccess)
+            st = ROCRAND_STATUS_INTERNAL_ERROR;
         if(st == ROCRAND_STATUS_SUCCESS)
             st = tune(*p);
         if(st == ROCRAND_STATUS_SUCCESS && hipEventCreateWithFlags(&p->done, hipEventDisableTiming) != hipSuccess)
@@ -531,7 +684,7 @@ rocrand_status ROCRANDAPI rocrand_batch_plan_create(rocrand_batch_plan*         
                 st = ROCRAND_STATUS_INTERNAL_ERROR;
             else if(st == ROCRAND_STATUS_SUCCESS)
             {
-                const hipError_t le = p->run_groups(cap);
+                const hipError_t le = run_groups(p->groups, p->tables, cap);
                 const hipError_t ce = hipStreamEndCapture(cap, &p->graph);
                 if(le != hipSuccess || ce != hipSuccess
                    || hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) != hipSuccess)
@@ -591,27 +744,43 @@ rocrand_status ROCRANDAPI rocrand_generate_batched(const rocrand_batch_stream* s
         return ROCRAND_STATUS_OUT_OF_RANGE;
     try
     {
-        rocrand_batch_plan_st p;
-        p.stream          = stream;
-        rocrand_status st = build(p, streams, count);
+        batch_layout   L;
+        rocrand_status st = build(L, streams, count);
         if(st != ROCRAND_STATUS_SUCCESS)
             return st;
-        for(auto& g : p.groups)
+        if(L.groups.empty())
+            return ROCRAND_STATUS_SUCCESS;
+        device_tables tables{};
+        int           device = 0;
+        if(get_device_tables(tables) != hipSuccess || hipGetDevice(&device) != hipSuccess)
+            return ROCRAND_STATUS_INTERNAL_ERROR;
+        for(auto& g : L.groups)
         {
             std::lock_guard<std::mutex> lock(g_tune_mtx);
-            const int  lg = g.bytes_out ? (int)std::log2((double)g.bytes_out) : 0;
-            auto       it = g_tuned.find(std::make_tuple(g.philox ? 1 : 0, (int)g.kind, lg));
+            auto                        it = g_tuned.find(tune_key(g));
             if(it != g_tuned.end())
                 g.blocks = it->second;
         }
-        if(p.run_groups(stream) != hipSuccess)
+        // Descriptors go through a pooled pinned buffer and a pooled device block, both
+        // released to the pool when the event recorded after the kernels has fired: the
+        // call returns without waiting for the device.
+        staging_pool::block blk;
+        hipEvent_t          ev = nullptr;
+        if(pool().acquire(device, L.bytes, blk, ev) != hipSuccess)
+            return ROCRAND_STATUS_ALLOCATION_FAILED;
+        L.pack(blk.host, blk.dev);
+        const bool ok = hipMemcpyAsync(blk.dev, blk.host, L.bytes, hipMemcpyHostToDevice, stream) == hipSuccess
+                        && run_groups(L.groups, tables, stream) == hipSuccess;
+        if(hipEventRecord(ev, stream) != hipSuccess)
         {
+            // the block cannot be tracked: wait for the stream, then recycle it
             (void)hipStreamSynchronize(stream);
-            return ROCRAND_STATUS_LAUNCH_FAILURE;
+            (void)hipEventRecord(ev, 0);
+            (void)hipEventSynchronize(ev);
         }
-        // the plan's device buffers are released when this returns: wait for the kernels
-        if(hipStreamSynchronize(stream) != hipSuccess)
-            return ROCRAND_STATUS_INTERNAL_ERROR;
+        pool().release_after(blk, ev);
+        if(!ok)
+            return ROCRAND_STATUS_LAUNCH_FAILURE;
     }
     catch(const std::bad_alloc&)
     {

@@ -7,9 +7,10 @@ server loop never waits for the device while work is pending:
   are done; their memory returns to the arena) and timestamps their requests;
 * it then admits the queued requests, first come first served, whose outputs fit the
   free arena memory (one pre-allocated HBM budget, first-fit with coalescing; a request
-  larger than the whole budget is rejected), and issues them as one batch: one batch
-  plan (include/rocrand_batched.h) for the Philox4x32-10 / MRG32k3a requests, one
-  quasi-random generator call per Sobol64 request, then a completion event;
+  larger than the whole budget is rejected), and issues them as one batch: one
+  asynchronous batched call (rocrand_generate_batched, include/rocrand_batched.h) for the
+  Philox4x32-10 / MRG32k3a requests, one quasi-random generator call per Sobol64
+  request, then a completion event;
 * consecutive batches alternate between two device streams, so a batch admitted while
   earlier ones run overlaps with them on the device.
 
@@ -19,6 +20,8 @@ include/rocrand_batched.h (Sobol64: a 16-dimensional generator at offset seed mo
 """
 from __future__ import annotations
 
+import bisect
+import collections
 import time
 from dataclasses import dataclass, field
 
@@ -82,7 +85,7 @@ class Arena:
 
     def __init__(self, size: int):
         self.size = size
-        self.free = [(0, size)]  # sorted (start, length)
+        self.free = [(0, size)]  # sorted (start, length), no two adjacent
 
     def alloc(self, n: int):
         for i, (s, ln) in enumerate(self.free):
@@ -95,22 +98,22 @@ class Arena:
         return None
 
     def release(self, start: int, n: int):
-        self.free.append((start, n))
-        self.free.sort()
-        merged = []
-        for s, ln in self.free:
-            if merged and merged[-1][0] + merged[-1][1] == s:
-                merged[-1] = (merged[-1][0], merged[-1][1] + ln)
-            else:
-                merged.append((s, ln))
-        self.free = merged
+        """Returns [start, start + n) to the free list, merged with its free neighbours."""
+        free = self.free
+        i = bisect.bisect_left(free, (start, 0))
+        if i < len(free) and start + n == free[i][0]:  # merge with the next range
+            n += free[i][1]
+            free.pop(i)
+        if i > 0 and free[i - 1][0] + free[i - 1][1] == start:  # and with the previous one
+            free[i - 1] = (free[i - 1][0], free[i - 1][1] + n)
+        else:
+            free.insert(i, (start, n))
 
 
 @dataclass
 class _Inflight:
     reqs: list
     event: object
-    plan: object
     regions: list
 
 
@@ -119,6 +122,7 @@ class Server:
         self.torch = torch
         self.budget = budget_bytes
         self.memory = torch.empty(budget_bytes, dtype=torch.uint8, device="cuda")
+        self.base = self.memory.data_ptr()
         self.arena = Arena(budget_bytes)
         self.streams = [torch.cuda.Stream() for _ in range(streams)]
         self.turn = 0
@@ -128,23 +132,21 @@ class Server:
         return self.memory[start:start + request_bytes(req)].view(dt)
 
     def issue(self, reqs, regions):
-        """Queue one batch on the next stream; returns (event, plan, views)."""
+        """Queue one batch on the next stream; returns its completion event."""
         torch = self.torch
         stream = self.streams[self.turn % len(self.streams)]
         self.turn += 1
-        views = [self._view(s, r) for s, r in zip(regions, reqs)]
-        plan = None
-        streams = batch_streams(reqs, [v.data_ptr() for v in views])
+        base = self.base
+        streams = batch_streams(reqs, [base + s for s in regions])
         with torch.cuda.stream(stream):
             if streams:
-                plan = R.BatchPlan(streams, stream)
-                plan.launch()
-            for r, v in zip(reqs, views):
+                R.generate_batched(streams, stream)
+            for r, s in zip(reqs, regions):
                 if r.rng_type == R.SOBOL64:
-                    write_sobol(r, v, stream)
+                    write_sobol(r, self._view(s, r), stream)
             ev = torch.cuda.Event()
             ev.record(stream)
-        return ev, plan, views
+        return ev
 
     def run_batch(self, reqs):
         """Writes `reqs` as one batch and waits for it (outputs stay valid until the next call)."""
@@ -152,10 +154,9 @@ class Server:
         regions = [self.arena.alloc(padded(request_bytes(r))) for r in reqs]
         if any(s is None for s in regions):
             raise ValueError("batch exceeds the budget")
-        ev, plan, views = self.issue(reqs, regions)
+        ev = self.issue(reqs, regions)
+        views = [self._view(s, r) for s, r in zip(regions, reqs)]
         ev.synchronize()
-        if plan is not None:
-            plan.close()
         self.arena = Arena(self.budget)
         return views
 
@@ -163,7 +164,7 @@ class Server:
         """Open-loop run: request i becomes visible at its arrival time."""
         res = Result()
         i = 0
-        queue, inflight = [], []
+        queue, inflight = collections.deque(), []
         t0 = time.perf_counter()
         while i < len(requests) or queue or inflight:
             now = time.perf_counter() - t0
@@ -175,8 +176,6 @@ class Server:
                     for r in b.reqs:
                         res.latencies_s.append(done - r.arrival_s)
                         res.bytes += request_bytes(r)
-                    if b.plan is not None:
-                        b.plan.close()
                     for s, r in zip(b.regions, b.reqs):
                         self.arena.release(s, padded(request_bytes(r)))
                 else:
@@ -195,11 +194,11 @@ class Server:
                 s = self.arena.alloc(padded(request_bytes(queue[0])))
                 if s is None:
                     break
-                batch.append(queue.pop(0))
+                batch.append(queue.popleft())
                 regions.append(s)
             if batch:
-                ev, plan, _ = self.issue(batch, regions)
-                inflight.append(_Inflight(batch, ev, plan, regions))
+                ev = self.issue(batch, regions)
+                inflight.append(_Inflight(batch, ev, regions))
                 res.iterations += 1
                 res.max_batch = max(res.max_batch, len(batch))
                 res.max_inflight = max(res.max_inflight, len(inflight))

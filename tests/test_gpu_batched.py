@@ -104,6 +104,34 @@ def test_one_shot_equals_plan():
     plan.close()
 
 
+def test_one_shot_async_repeated():
+    """Back-to-back one-shot calls without host waits (their descriptor blocks stay in use
+    until each call completes, then are recycled): every call's outputs equal the plain
+    calls, also for calls issued after the pool has recycled blocks."""
+    stream = torch.cuda.Stream()
+    cases = []
+    for seed in (5, 6, 7):
+        specs, shifts = _mix(24, seed, 4.0 + 0.3 * seed)
+        _, pviews = _buffers(specs, shifts)
+        for s, pv in zip(specs, pviews):
+            _plain(s, pv)
+        cases.append((specs, shifts, pviews))
+    torch.cuda.synchronize()
+    for rnd in range(2):
+        outs = []
+        with torch.cuda.stream(stream):
+            for rep in range(8):
+                specs, shifts, _ = cases[rep % 3]
+                _, views = _buffers(specs, shifts)
+                R.generate_batched(_streams(specs, views), stream)
+                outs.append((rep % 3, views))
+        stream.synchronize()
+        for ci, views in outs:
+            for i, (v, pv) in enumerate(zip(views, cases[ci][2])):
+                assert np.array_equal(v.cpu().numpy().view(np.uint8), pv.cpu().numpy().view(np.uint8)), \
+                    f"round {rnd} case {ci} stream {i}"
+
+
 def test_batched_against_oracle():
     specs, shifts = _mix(16, 4, 4.2)
     _, views = _buffers(specs, shifts)

@@ -1,5 +1,6 @@
 """CPU tests of the serving loop's host logic: the arena allocator and the request mixes."""
 import math
+import random
 
 from rocrand_amd import rocrand as R
 from rocrand_amd.serving import Arena, padded
@@ -26,6 +27,44 @@ def test_arena_first_fit_and_coalescing():
     a.release(z, 256)
     assert a.free == [(0, 1024)]
     assert padded(1) == 256 and padded(256) == 256 and padded(257) == 512
+
+
+def test_arena_matches_bitmap_model():
+    """Random alloc/release sequences: first-fit positions and the free list equal those of
+    a per-byte bitmap (free list = maximal free runs)."""
+    rs = random.Random(1)
+    for _ in range(100):
+        size = rs.randrange(1, 300)
+        a, used, bitmap = Arena(size), [], [0] * size
+        for _ in range(150):
+            if used and rs.random() < 0.5:
+                s, n = used.pop(rs.randrange(len(used)))
+                a.release(s, n)
+                bitmap[s:s + n] = [0] * n
+            else:
+                n = rs.randrange(1, 40)
+                want, run = None, 0
+                for j in range(size):
+                    run = run + 1 if bitmap[j] == 0 else 0
+                    if run == n:
+                        want = j - n + 1
+                        break
+                got = a.alloc(n)
+                assert got == want
+                if got is not None:
+                    used.append((got, n))
+                    bitmap[got:got + n] = [1] * n
+            runs, j = [], 0
+            while j < size:
+                if bitmap[j] == 0:
+                    k = j
+                    while k < size and bitmap[k] == 0:
+                        k += 1
+                    runs.append((j, k - j))
+                    j = k
+                else:
+                    j += 1
+            assert a.free == runs
 
 
 def test_batch_mix_shape():
