@@ -115,6 +115,18 @@ unsigned int poisson_table(double lambda, unsigned int& offset, std::vector<doub
     return size;
 }
 
+std::vector<uint64_t> pack_alias(const std::vector<double>& prob, const std::vector<unsigned int>& alias)
+{
+    std::vector<uint64_t> out(prob.size());
+    for(size_t i = 0; i < prob.size(); i++)
+    {
+        // prob in [0, 1]: the scaling by 2^32 is exact and the ceiling fits 33 bits
+        const uint64_t thr = (uint64_t)std::ceil(std::ldexp(prob[i], 32));
+        out[i]             = thr | ((uint64_t)alias[i] << 33);
+    }
+    return out;
+}
+
 // Scratch for the alias probabilities that the CDF method computes and drops.
 std::vector<double>& alias_unused_prob()
 {
@@ -133,6 +145,8 @@ poisson_cache::~poisson_cache()
     (void)hipFree(m_dalias);
     (void)hipHostFree(m_hprob);
     (void)hipHostFree(m_halias);
+    (void)hipFree(m_dpacked);
+    (void)hipHostFree(m_hpacked);
 }
 
 rocrand_status poisson_cache::get(double lambda, hipStream_t stream, dist_params& p)
@@ -148,6 +162,8 @@ rocrand_status poisson_cache::get(double lambda, hipStream_t stream, dist_params
            || hipMalloc(&m_dalias, POISSON_MAX_SIZE * sizeof(unsigned int)) != hipSuccess
            || hipHostMalloc(&m_hprob, POISSON_MAX_SIZE * sizeof(double)) != hipSuccess
            || hipHostMalloc(&m_halias, POISSON_MAX_SIZE * sizeof(unsigned int)) != hipSuccess
+           || hipMalloc(&m_dpacked, POISSON_MAX_SIZE * sizeof(uint64_t)) != hipSuccess
+           || hipHostMalloc(&m_hpacked, POISSON_MAX_SIZE * sizeof(uint64_t)) != hipSuccess
            || hipEventCreateWithFlags(&m_copied, hipEventDisableTiming) != hipSuccess)
             return ROCRAND_STATUS_ALLOCATION_FAILED;
     }
@@ -169,9 +185,17 @@ rocrand_status poisson_cache::get(double lambda, hipStream_t stream, dist_params
             return ROCRAND_STATUS_INTERNAL_ERROR;
         memcpy(m_hprob, prob.data(), m_size * sizeof(double));
         memcpy(m_halias, alias.data(), m_size * sizeof(unsigned int));
+        if(!cdf_method)
+        {
+            const std::vector<uint64_t> packed = pack_alias(prob, alias);
+            memcpy(m_hpacked, packed.data(), m_size * sizeof(uint64_t));
+        }
         if(hipMemcpyAsync(m_dprob, m_hprob, m_size * sizeof(double), hipMemcpyHostToDevice, stream) != hipSuccess
            || hipMemcpyAsync(m_dalias, m_halias, m_size * sizeof(unsigned int), hipMemcpyHostToDevice, stream)
                   != hipSuccess
+           || (!cdf_method
+               && hipMemcpyAsync(m_dpacked, m_hpacked, m_size * sizeof(uint64_t), hipMemcpyHostToDevice, stream)
+                      != hipSuccess)
            || hipEventRecord(m_copied, stream) != hipSuccess)
             return ROCRAND_STATUS_INTERNAL_ERROR;
         m_lambda = lambda;
@@ -181,6 +205,7 @@ rocrand_status poisson_cache::get(double lambda, hipStream_t stream, dist_params
     p.pt.prob   = cdf_method ? nullptr : m_dprob;
     p.pt.alias  = m_dalias;
     p.pt.cdf    = cdf_method ? m_dprob : nullptr;
+    p.pt.packed = cdf_method ? nullptr : m_dpacked;
     return ROCRAND_STATUS_SUCCESS;
 }
 

@@ -245,7 +245,8 @@ struct batch_layout
     std::vector<group>                     groups;
     std::vector<std::vector<double>>       prob;
     std::vector<std::vector<unsigned int>> alias;
-    std::vector<size_t>                    prob_at, alias_at;
+    std::vector<std::vector<uint64_t>>     packed;
+    std::vector<size_t>                    prob_at, alias_at, packed_at;
     size_t                                 bytes = 0;
 
     static size_t align(size_t x) { return (x + 255) / 256 * 256; }
@@ -259,6 +260,8 @@ struct batch_layout
             at = align(at + prob[i].size() * sizeof(double));
             alias_at.push_back(at);
             at = align(at + alias[i].size() * sizeof(unsigned int));
+            packed_at.push_back(at);
+            at = align(at + packed[i].size() * sizeof(uint64_t));
         }
         for(auto& g : groups)
         {
@@ -278,6 +281,7 @@ struct batch_layout
         {
             memcpy(host + prob_at[i], prob[i].data(), prob[i].size() * sizeof(double));
             memcpy(host + alias_at[i], alias[i].data(), alias[i].size() * sizeof(unsigned int));
+            memcpy(host + packed_at[i], packed[i].data(), packed[i].size() * sizeof(uint64_t));
         }
         auto fix = [&](dist_params& prm) {
             if(prm.pt.size == 0)
@@ -285,6 +289,7 @@ struct batch_layout
             const size_t t = reinterpret_cast<uintptr_t>(prm.pt.prob);
             prm.pt.prob    = reinterpret_cast<const double*>(dev + prob_at[t]);
             prm.pt.alias   = reinterpret_cast<const unsigned int*>(dev + alias_at[t]);
+            prm.pt.packed  = reinterpret_cast<const uint64_t*>(dev + packed_at[t]);
         };
         for(auto& g : groups)
         {
@@ -486,6 +491,7 @@ rocrand_status build(batch_layout& L, const rocrand_batch_stream* streams, size_
                     t.offset = off;
                     // the table's index until the image is packed
                     t.prob = reinterpret_cast<const double*>(static_cast<uintptr_t>(L.prob.size()));
+                    L.packed.push_back(pack_alias(prob, alias));
                     L.prob.push_back(std::move(prob));
                     L.alias.push_back(std::move(alias));
                     it = tables.emplace(b.a, t).first;

@@ -317,7 +317,8 @@ struct alias_table
     unsigned int        offset;
     const unsigned int* alias;
     const double*       prob;
-    const double*       cdf; // cumulative table (quasi-random generators)
+    const double*       cdf;    // cumulative table (quasi-random generators)
+    const uint64_t*     packed; // ceil(prob * 2^32) | alias << 33, see discrete_alias_u32
 };
 
 __device__ __forceinline__ unsigned int discrete_alias(double x, const alias_table& t)
@@ -327,6 +328,19 @@ __device__ __forceinline__ unsigned int discrete_alias(double x, const alias_tab
     const double       y   = __dsub_rn(nx, fnx);
     const unsigned int i   = (unsigned int)fnx;
     return t.offset + (y < t.prob[i] ? i : t.alias[i]);
+}
+
+// discrete_alias for the draw x = r 2^-32 of a 32-bit word, in integers: size * x is
+// exact in double (size <= 2^12), so floor(nx) = (size r) >> 32 and
+// nx - floor(nx) = ((size r) mod 2^32) 2^-32 exactly, and for an integer lo,
+// lo 2^-32 < prob  <=>  lo < ceil(prob 2^32).  One table load per sample.
+__device__ __forceinline__ unsigned int discrete_alias_u32(uint32_t r, const alias_table& t)
+{
+    const uint64_t nx = (uint64_t)t.size * r;
+    const uint32_t i  = (uint32_t)(nx >> 32);
+    const uint64_t e  = t.packed[i];
+    const uint64_t lo = (uint32_t)nx;
+    return t.offset + (lo < (e & ((1ull << 33) - 1)) ? i : (uint32_t)(e >> 33));
 }
 
 // ---------------------------------------------------------------- functors
@@ -450,7 +464,7 @@ RR_DIST(word_family::w32, out_kind::poisson, 1, 1, uint32_t)
         out[0] = (uint32_t)round(__dadd_rn(__dmul_rn(p.sqrt_lambda, nd), p.lambda)) ;
         return;
     }
-    out[0] = discrete_alias(__dmul_rn((double)r, R_2POW32_INV_DBL), p.pt);
+    out[0] = discrete_alias_u32(r, p.pt);
 } };
 
 // normal_f64 / lognormal_f64: four words per pair (default), or one word through
@@ -679,7 +693,7 @@ RR_DIST(word_family::w64, out_kind::poisson, 1, 1, uint32_t)
             out[0] = (uint32_t)round(__dadd_rn(__dmul_rn(p.sqrt_lambda, nd), p.lambda));          \
             return;                                                                              \
         }                                                                                        \
-        out[0] = discrete_alias(__dmul_rn((double)r, R_2POW32_INV_DBL), p.pt);                   \
+        out[0] = discrete_alias_u32(r, p.pt);                                                    \
     } };
 
 RR_MRG_DISTS(word_family::mrg32, 32)

@@ -53,6 +53,8 @@ private:
     unsigned int* m_dalias = nullptr;
     double*       m_hprob  = nullptr; // pinned staging
     unsigned int* m_halias = nullptr;
+    uint64_t*     m_dpacked = nullptr; // packed alias table (alias method)
+    uint64_t*     m_hpacked = nullptr;
     hipEvent_t    m_copied = nullptr;
 };
 
@@ -62,6 +64,8 @@ std::vector<double> poisson_pmf(double lambda, unsigned int& size, unsigned int&
 // poisson.hpp:110-161 (pmf) and discrete.hpp:189-265 (normalize, Vose alias).
 unsigned int poisson_table(double lambda, unsigned int& offset, std::vector<double>& prob,
                            std::vector<unsigned int>& alias);
+// Packed alias entries for integer sampling: ceil(prob[i] 2^32) | alias[i] << 33.
+std::vector<uint64_t> pack_alias(const std::vector<double>& prob, const std::vector<unsigned int>& alias);
 // Alias table of an arbitrary probability vector (discrete.hpp:97-140).
 void alias_table_from(const std::vector<double>& p, unsigned int size, std::vector<double>& prob,
                       std::vector<unsigned int>& alias, std::vector<double>* cdf);

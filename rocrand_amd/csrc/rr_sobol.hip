@@ -26,7 +26,8 @@ __device__ __forceinline__ float roc_f_erfinv(float x)
 {
     const float sgn = (x < 0.0f) ? -1.0f : 1.0f;
     x               = __fmul_rn(__fsub_rn(1.0f, x), __fadd_rn(1.0f, x));
-    const float lnx = logf_acc(x);
+    // (1 - x)(1 + x) is a positive normal float or 0 here (|x| <= 1)
+    const float lnx = x > 0.0f ? logf_tab(x) : logf_acc(x);
     if(isnan(lnx))
         return 1.0f;
     if(isinf(lnx))
