@@ -387,6 +387,15 @@ enum class word_family : int
 template<word_family F, out_kind K, bool NormalDoubleOneWord = false>
 struct dist;
 
+// The output kind of a transform type.
+template<class D>
+struct dist_kind;
+template<word_family F, out_kind K, bool B>
+struct dist_kind<dist<F, K, B>>
+{
+    static constexpr out_kind value = K;
+};
+
 // Generic 32-bit engines (uniform.hpp:37-120, normal.hpp:45-108, log_normal.hpp:45-108).
 #define RR_DIST(F, K, IW, OW, T) \
     template<bool B>              \

@@ -532,9 +532,13 @@ __host__ __device__ __forceinline__ void gf2_mat_vec(const uint32_t* __restrict_
 #pragma unroll
     for(int k = 0; k < N; k++)
         r[k] = 0;
+    // the word loop is unrolled so that v and r stay in registers; the bit loop is not
+    // (its rows are read as it goes)
+#pragma unroll
     for(int i = 0; i < N; i++)
     {
         const uint32_t vi = v[i];
+#pragma unroll 2
         for(int j = 0; j < 32; j++)
         {
             const uint32_t  b   = 0u - ((vi >> j) & 1u);

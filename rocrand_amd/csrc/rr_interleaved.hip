@@ -65,6 +65,16 @@ __global__ __launch_bounds__(256) void generate_kernel(interleaved_call c)
     alignas(16) out_t o[OUT];
     uint32_t     in[IN];
     size_t       index = id;
+    // The transforms (and table loads) of consecutive steps overlap when unrolled, which
+    // the 2 wavefronts per SIMD of an S-lane grid cannot provide; integer and uniform
+    // outputs are store-bound already.
+    constexpr out_kind ok     = dist_kind<D>::value;
+    constexpr int      UNROLL = (ok == out_kind::u8 || ok == out_kind::u16 || ok == out_kind::u32
+                                 || ok == out_kind::uniform_f32 || ok == out_kind::uniform_f64
+                                 || ok == out_kind::uniform_f16)
+                                    ? 1
+                                    : 4;
+#pragma unroll UNROLL
     for(; index < c.vec_n; index += S)
     {
 #pragma unroll
@@ -101,10 +111,12 @@ __global__ __launch_bounds__(256) void generate_kernel(interleaved_call c)
     g.store(c.states, eid);
 }
 
-// MRG engines: each engine's vec-steps are split into J chunks of C, one lane per
+// Long calls: each engine's vec-steps are split into J chunks of C, one lane per
 // (engine, chunk).  A lane jumps its copy of the engine state ahead by j*C*IN words
-// with the A^(2^i) tables (a handful of 3x3 modular products), so S*J lanes run the
-// serial recurrence instead of S: the per-engine dependency chain no longer limits the
+// (MRG: the A^(2^i) tables, a handful of 3x3 modular products; XORWOW / LFSR113: the
+// GF(2) 4^k powers, with C a power of 4 so that j*C*IN has at most three nonzero base-4
+// digits), so S*J lanes run the serial recurrence instead of S: the per-engine
+// dependency chain and the S-lane grid's two wavefronts per SIMD no longer limit the
 // store rate.  Lanes are ordered chunk-major, so a wavefront's 64 lanes are 64
 // consecutive engines of one chunk and each of its stores is one contiguous segment.
 // The lane whose range ends at the engine's last vec-step also writes the head/tail
@@ -119,8 +131,11 @@ __global__ __launch_bounds__(256) void generate_chunked_kernel(interleaved_call 
     constexpr int BYTES = OUT * (int)sizeof(out_t);
     static_assert(BYTES == 4 || BYTES == 8 || BYTES == 16, "vec size");
 
-    const uint32_t g  = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t id = g % S, j = g / S;
+    // S is a multiple of the block size: the chunk index is uniform per block, so the
+    // jump's digit loop and its table rows stay in scalar registers
+    constexpr uint32_t BPC = S / 256; // blocks per chunk
+    const uint32_t     j   = blockIdx.x / BPC;
+    const uint32_t     id  = (blockIdx.x % BPC) * 256 + threadIdx.x;
     if(j >= J)
         return;
     // vec-steps of this engine: vec indices id, id + S, ... below vec_n
@@ -128,12 +143,26 @@ __global__ __launch_bounds__(256) void generate_chunked_kernel(interleaved_call 
     const uint64_t k0     = (uint64_t)j * C;
     const uint64_t end_j  = steps / C < J ? steps / C : J - 1; // chunk that ends at `steps`
     const bool     ending = j == end_j;
-    if(k0 > steps && !ending)
-        return;
-    const uint32_t eid = (id + c.start) % S;
+    const bool     idle   = k0 > steps && !ending; // nothing to write in this chunk
+    const uint32_t eid    = (id + c.start) % S;
     ieng<K>        eng;
-    eng.load(c.states, eid);
-    eng.jump(K == inter_kind::mrg32k3a ? t.mrg32_step : t.mrg31_step, k0 * IN);
+    if constexpr(K == inter_kind::xorwow || K == inter_kind::lfsr113)
+    {
+        // GF(2) jumps: k0 is the same for the whole block, which stages each matrix in
+        // LDS once; every thread takes part (barriers) before idle ones leave
+        __shared__ uint32_t mat[ieng<K>::LDS_WORDS];
+        eng.load(c.states, eid);
+        eng.advance_block(t, k0 * IN, mat);
+        if(idle)
+            return;
+    }
+    else
+    {
+        if(idle)
+            return;
+        eng.load(c.states, eid);
+        eng.advance(t, k0 * IN);
+    }
 
     out_t* const data     = static_cast<out_t*>(c.data);
     out_t* const vec_data = data + c.mis;
@@ -188,6 +217,30 @@ __global__ __launch_bounds__(256) void generate_chunked_kernel(interleaved_call 
 template<inter_kind K, class D>
 hipError_t launch_gen(const interleaved_call& c, hipStream_t s)
 {
+    if constexpr(K == inter_kind::xorwow || K == inter_kind::lfsr113)
+    {
+        // XORWOW / LFSR113 write cheap outputs at the store rate with S lanes, and the
+        // ALU-bound transforms (Box-Muller, exp) find enough independent work in the
+        // unrolled loop; Poisson's dependent table loads need more wavefronts: chunks.
+        if constexpr(dist_kind<D>::value == out_kind::poisson)
+        {
+            const uint64_t kmax = (c.vec_n + S - 1) / S;
+            if(kmax >= 64)
+            {
+                uint64_t C = 16;
+                while((kmax + C - 1) / C > 16)
+                    C *= 4;
+                const uint32_t J = (uint32_t)((kmax + C - 1) / C);
+                device_tables  t;
+                hipError_t     err = get_device_tables(t);
+                if(err != hipSuccess)
+                    return err;
+                hipLaunchKernelGGL((generate_chunked_kernel<K, D>), dim3(S / 256 * J), dim3(256), 0, s, c, t, J,
+                                   (uint32_t)C);
+                return hipGetLastError();
+            }
+        }
+    }
     if constexpr(K == inter_kind::mrg32k3a || K == inter_kind::mrg31k3p)
     {
         // chunks per engine: enough lanes to hide the recurrence's latency

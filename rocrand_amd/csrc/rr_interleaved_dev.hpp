@@ -92,6 +92,8 @@ struct ieng<inter_kind::mrg32k3a>
             }
     }
     __device__ void sub(const device_tables& t, uint32_t e) { mrg_sub_jump<32, 209, 22853>(t.mrg32_subr, e, s); }
+    // k steps ahead (rocrand_mrg32k3a.h:246-263)
+    __device__ void advance(const device_tables& t, uint64_t k) { jump(t.mrg32_step, k); }
     __device__ void init(const device_tables& t, uint32_t e, uint64_t off)
     {
         sub(t, e);
@@ -114,6 +116,7 @@ struct ieng<inter_kind::mrg31k3p> : ieng<inter_kind::mrg32k3a>
             }
     }
     __device__ void sub(const device_tables& t, uint32_t e) { mrg_sub_jump<31, 1, 21069>(t.mrg31_subr, e, s); }
+    __device__ void advance(const device_tables& t, uint64_t k) { jump(t.mrg31_step, k); }
     __device__ void init(const device_tables& t, uint32_t e, uint64_t off)
     {
         sub(t, e);
@@ -131,6 +134,28 @@ __device__ void gf2_jump(uint32_t* v, const uint32_t* mats, uint64_t n)
         const uint32_t d = (uint32_t)(n & 3u);
         for(uint32_t i = 0; i < d; i++)
             gf2_mat_vec<N>(mats + (size_t)k * 32 * N * N, v);
+    }
+}
+
+// gf2_jump for a whole block whose lanes share `n` (block-uniform): each 4^k power is
+// staged once in LDS (`lds`: 32 N^2 words) and read from there by every lane, instead
+// of every wavefront fetching the same rows through the scalar or vector caches.
+// Every thread of the block must call it.
+template<int N>
+__device__ void gf2_jump_block(uint32_t* v, const uint32_t* mats, uint64_t n, uint32_t* lds)
+{
+    constexpr int SZ = 32 * N * N;
+    for(int k = 0; n > 0; n >>= 2, k++)
+    {
+        const uint32_t d = (uint32_t)(n & 3u);
+        if(d == 0)
+            continue;
+        __syncthreads(); // the previous matrix is no longer read
+        for(int w = threadIdx.x; w < SZ; w += blockDim.x)
+            lds[w] = mats[(size_t)k * SZ + w];
+        __syncthreads();
+        for(uint32_t i = 0; i < d; i++)
+            gf2_mat_vec<N>(lds, v);
     }
 }
 
@@ -166,6 +191,19 @@ struct ieng<inter_kind::xorwow>
         gf2_jump<5>(s.x, t.xorwow_jump, off);
         s.d += (uint32_t)off * 362437u; // rocrand_xorwow.h:134-145
     }
+    // k steps ahead: the 4^j powers for the xorshift part, the Weyl counter linearly
+    __device__ void advance(const device_tables& t, uint64_t k)
+    {
+        gf2_jump<5>(s.x, t.xorwow_jump, k);
+        s.d += (uint32_t)k * 362437u;
+    }
+    static constexpr int LDS_WORDS = 32 * 5 * 5;
+    // the same for a block sharing k (every thread calls it)
+    __device__ void advance_block(const device_tables& t, uint64_t k, uint32_t* lds)
+    {
+        gf2_jump_block<5>(s.x, t.xorwow_jump, k, lds);
+        s.d += (uint32_t)k * 362437u;
+    }
 };
 
 template<>
@@ -196,6 +234,12 @@ struct ieng<inter_kind::lfsr113>
         // lfsr113.hpp:45-62: the step count travels as an unsigned int.
         gf2_jump<4>(s.z, t.lfsr_seq, (uint32_t)e);
         gf2_jump<4>(s.z, t.lfsr_jump, (uint32_t)off);
+    }
+    __device__ void advance(const device_tables& t, uint64_t k) { gf2_jump<4>(s.z, t.lfsr_jump, k); }
+    static constexpr int LDS_WORDS = 32 * 4 * 4;
+    __device__ void advance_block(const device_tables& t, uint64_t k, uint32_t* lds)
+    {
+        gf2_jump_block<4>(s.z, t.lfsr_jump, k, lds);
     }
 };
 

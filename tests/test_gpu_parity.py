@@ -158,13 +158,21 @@ def test_orderings_same_stream(ename, order):
     assert np.array_equal(got[0][0], want)
 
 
-@pytest.mark.parametrize("ename", ["mrg32k3a", "mrg31k3p"])
-def test_mrg_large_calls(ename):
-    """Calls long enough (more than 64 vec-steps per engine) to take the chunked MRG kernel,
+LARGE_CASES = {
+    # MRG: every kind chunks above 64 vec-steps per engine
+    "mrg": [("u32", (1 << 24) + 3, 1), ("uniform_f32", 1 << 25, 0), ("normal_f32", (1 << 24) + 1, 1),
+            ("uniform_f64", (1 << 23) + 5, 1)],
+    # XORWOW / LFSR113: Poisson chunks (C a power of 4); the others stay on S lanes
+    "gf2": [("poisson", (1 << 23) + 7, 0), ("poisson", (1 << 26) + 5, 3), ("normal_f32", (1 << 24) + 1, 1)],
+}
+
+
+@pytest.mark.parametrize("ename", ["mrg32k3a", "mrg31k3p", "xorwow", "lfsr113"])
+def test_large_chunked_calls(ename):
+    """Calls long enough (at least 64 vec-steps per engine) to take the chunked kernel,
     misaligned and ragged, two per generator, against the oracle."""
     rng = ENGINES[ename]
-    for kind, n, shift in [("u32", (1 << 24) + 3, 1), ("uniform_f32", 1 << 25, 0), ("normal_f32", (1 << 24) + 1, 1),
-                           ("uniform_f64", (1 << 23) + 5, 1)]:
+    for kind, n, shift in LARGE_CASES["mrg" if ename.startswith("mrg") else "gf2"]:
         print(f"case {ename} {kind} n={n} shift={shift}", flush=True)
         got = _gpu_stream(rng, kind, 2024, 1048583, n, shift)
         og = O.OracleGen(rng, seed=2024, offset=1048583)
