@@ -45,22 +45,19 @@ constexpr float  R_SQRT2          = 1.4142135f;
 //        table of rr_mathtab.hpp, |z/c_i - 1| < 2^-7, series to r^6 (relative error < 2^-44);
 //   log (erfinv argument, any float): x = 2^e m, m in [sqrt(1/2), sqrt(2)),
 //        ln m = 2 atanh(f), f = (m-1)/(m+1), |f| < 0.172, series to f^21;
-//   sin/cos for x in [0, 8): Cody-Waite reduction by pi/2 (two parts, fused), Taylor
-//        series to r^13 / r^14 on |r| <= pi/4;
+//   sin/cos for x in [0, 8): reduction by pi/32 (two parts, fused), sin/cos(k pi/32) from
+//        rr_mathtab.hpp and Taylor series to r^7 / r^6 on |r| <= pi/64;
 //   expf_acc: reduction by ln2/32 (two parts), 2^(j/32) from rr_mathtab.hpp, Taylor series
 //        of exp(r) - 1 to r^5 on |r| <= ln2/64, scaling by 2^e in double.
 // A host C twin of these routines (exact division in place of the refined reciprocal)
-// rounds to the correctly rounded float result for every float in [2^-32, 1] (logf_tab
-// and log), every float in (0, 7] but one (sin; that one lies within 2^-48 of a rounding
-// boundary), every float in (0, 7] (cos) and every float in (-105, 90) but two (exp).
+// rounds to the correctly rounded float result for every float in [2^-32, 1] but one
+// (logf_tab; that one lies within 2^-48 of a rounding boundary), every float in (0, 7]
+// (sin, cos) and every float in (-105, 90) but two (exp).
 // So each differs from the host libm's float result by at most the host's own error
 // (< 1 ulp).
 constexpr double RR_LN2_HI  = 6.93147180369123816490e-01; // ln 2, top 32 bits
 constexpr double RR_LN2_LO  = 1.90821492927058770002e-10; // ln 2 - RR_LN2_HI
 constexpr double RR_LN2     = 6.93147180559945286227e-01; // ln 2 rounded to double
-constexpr double RR_PIO2_1  = 1.57079632679489655800e+00; // pi/2 rounded to double
-constexpr double RR_PIO2_2  = 6.12323399573676603587e-17; // pi/2 - RR_PIO2_1
-constexpr double RR_2_OV_PI = 6.36619772367581382433e-01;
 
 // ln u for a positive normal float u (the Box-Muller uniforms lie in [2^-32, 1]).
 __device__ __forceinline__ float logf_tab(float u)
@@ -115,35 +112,41 @@ __device__ __forceinline__ double log_dbl(double x)
     return y;
 }
 
-// sin and cos of x in [0, 8), rounded to float; the quadrant fix-up is done on the
-// rounded values (rounding commutes with negation).
+// sin and cos of x in [0, 8), rounded to float: x = q pi/32 + r (two-part reduction,
+// |r| <= pi/64), then the angle-addition formulas with sin/cos(k pi/32), k = q mod 64,
+// from rr_mathtab.hpp and Taylor series of sin r (to r^7) and cos r (to r^6).  Where
+// sin x or cos x is near 0 the table entry is an exact 0 / +-1, so the sum does not cancel.
 __device__ __forceinline__ void sincosf_acc(float xf, float* s, float* c)
 {
-    const double x  = (double)xf;
-    const double q  = __builtin_rint(x * RR_2_OV_PI);
-    double       r  = __builtin_fma(-q, RR_PIO2_1, x);
-    r               = __builtin_fma(-q, RR_PIO2_2, r);
-    const double r2 = r * r;
-    double       ps = 1.0 / 6227020800.0;
-    ps              = __builtin_fma(ps, r2, -1.0 / 39916800.0);
-    ps              = __builtin_fma(ps, r2, 1.0 / 362880.0);
-    ps              = __builtin_fma(ps, r2, -1.0 / 5040.0);
-    ps              = __builtin_fma(ps, r2, 1.0 / 120.0);
-    ps              = __builtin_fma(ps, r2, -1.0 / 6.0);
-    const float  sn = (float)__builtin_fma(r * r2, ps, r);
-    double       pc = -1.0 / 87178291200.0;
-    pc              = __builtin_fma(pc, r2, 1.0 / 479001600.0);
-    pc              = __builtin_fma(pc, r2, -1.0 / 3628800.0);
-    pc              = __builtin_fma(pc, r2, 1.0 / 40320.0);
-    pc              = __builtin_fma(pc, r2, -1.0 / 720.0);
-    pc              = __builtin_fma(pc, r2, 1.0 / 24.0);
-    pc              = __builtin_fma(pc, r2, -0.5);
-    const float  cs = (float)__builtin_fma(r2, pc, 1.0);
-    const int    qi = (int)q & 3;
-    const float  a  = (qi & 1) ? cs : sn;
-    const float  b  = (qi & 1) ? sn : cs;
-    *s              = (qi & 2) ? -a : a;
-    *c              = ((qi + 1) & 2) ? -b : b;
+    const double  x  = (double)xf;
+    const double  q  = __builtin_rint(x * RR_SC_INV_STEP);
+    const int     k  = (int)q & ((1 << RR_SCTAB_BITS) - 1);
+    double        r  = __builtin_fma(-q, RR_SC_STEP_HI, x);
+    r                = __builtin_fma(-q, RR_SC_STEP_LO, r);
+    const double2 t  = rr_sctab[k];
+    const double  r2 = r * r;
+    double        ps = __builtin_fma(r2, -1.0 / 5040, 1.0 / 120);
+    ps               = __builtin_fma(ps, r2, -1.0 / 6);
+    const double  sr = __builtin_fma(r * r2, ps, r);
+    double        pc = __builtin_fma(r2, -1.0 / 720, 1.0 / 24);
+    pc               = __builtin_fma(pc, r2, -0.5);
+    const double  cr = __builtin_fma(r2, pc, 1.0);
+    *s               = (float)__builtin_fma(t.x, cr, t.y * sr);
+    *c               = (float)__builtin_fma(t.y, cr, -(t.x * sr));
+}
+
+// sqrtf for 0 and positive normal floats: the hardware square root (within 1 ulp) and
+// the two fused-multiply-add checks of its neighbours that make it correctly rounded --
+// the same instructions as the compiler's sqrtf without its rescaling of tiny inputs
+// (below 2^-96) and of infinities, which the Box-Muller radius (0 or >= 2^-24) never is.
+__device__ __forceinline__ float sqrtf_radius(float x)
+{
+    const float s  = __builtin_amdgcn_sqrtf(x);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+    float       r  = __builtin_fmaf(-sm, s, x) <= 0.0f ? sm : s;
+    r              = __builtin_fmaf(-sp, s, x) > 0.0f ? sp : r;
+    return r;
 }
 
 __device__ __forceinline__ float logf_acc(float x) { return (float)log_dbl((double)x); }
@@ -234,7 +237,7 @@ __device__ __forceinline__ void box_muller(uint32_t x, uint32_t y, float& r0, fl
 {
     const float u = uniform_f32(x);
     const float v = __fadd_rn(R_2POW32_INV_2PI, __fmul_rn((float)y, R_2POW32_INV_2PI));
-    const float s = sqrtf(__fmul_rn(-2.0f, logf_tab(u)));
+    const float s = sqrtf_radius(__fmul_rn(-2.0f, logf_tab(u)));
     float       sn, cs;
     sincosf_acc(v, &sn, &cs);
     r0 = __fmul_rn(sn, s);
@@ -260,7 +263,7 @@ __device__ __forceinline__ void box_muller_half(uint32_t x16, uint32_t y16, floa
 {
     const float u = __fadd_rn(R_2POW16_INV, __fmul_rn((float)(x16 & 0xffffu), R_2POW16_INV));
     const float v = __fadd_rn(R_2POW16_INV_2PI, __fmul_rn((float)(y16 & 0xffffu), R_2POW16_INV_2PI));
-    const float s = sqrtf(__fmul_rn(-2.0f, logf_tab(u)));
+    const float s = sqrtf_radius(__fmul_rn(-2.0f, logf_tab(u)));
     float       sn, cs;
     sincosf_acc(v, &sn, &cs);
     h0 = __half2float(__float2half(__fmul_rn(sn, s)));
@@ -272,7 +275,7 @@ __device__ __forceinline__ void mrg_box_muller(uint32_t x, uint32_t y, float& r0
 {
     const float u = mrg_f32<MRG>(x);
     const float v = __fmul_rn(mrg_f32<MRG>(y), R_2PI);
-    const float s = sqrtf(__fmul_rn(-2.0f, logf_tab(u)));
+    const float s = sqrtf_radius(__fmul_rn(-2.0f, logf_tab(u)));
     float       sn, cs;
     sincosf_acc(v, &sn, &cs);
     r0 = __fmul_rn(sn, s);

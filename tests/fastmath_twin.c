@@ -6,8 +6,7 @@
  *
  * usage: fastmath_twin STRIDE
  * Prints, for every STRIDE-th float of each domain, the number of results that differ
- * from the correctly rounded one (the long double libm result rounded to float, which
- * agrees with double-precision libm on all of these):
+ * from the correctly rounded one (the long double libm result rounded to float):
  *   log  [2^-32, 1]   sin, cos  (0, 7]   exp  (-105, 90)
  */
 #include <math.h>
@@ -19,9 +18,6 @@
 #include "mathtab_twin.h"
 
 static const double LN2        = 6.93147180559945286227e-01;
-static const double PIO2_1     = 1.57079632679489655800e+00;
-static const double PIO2_2     = 6.12323399573676603587e-17;
-static const double TWO_OVER_PI = 6.36619772367581382433e-01;
 
 static inline uint32_t fbits(float x) { uint32_t b; memcpy(&b, &x, 4); return b; }
 static inline float bitsf(uint32_t b) { float x; memcpy(&x, &b, 4); return x; }
@@ -43,29 +39,19 @@ static float logf_tab(float u)
 
 static void sincosf_acc(float xf, float* s, float* c)
 {
-    const double x = xf, q = rint(x * TWO_OVER_PI);
-    double r = fma(-q, PIO2_1, x);
-    r = fma(-q, PIO2_2, r);
+    const double x = xf, q = rint(x * SC_INV_STEP);
+    const int k = (int)q & ((1 << SCTAB_BITS) - 1);
+    double r = fma(-q, SC_STEP_HI, x);
+    r = fma(-q, SC_STEP_LO, r);
     const double r2 = r * r;
-    double ps = 1.0 / 6227020800.0;
-    ps = fma(ps, r2, -1.0 / 39916800.0);
-    ps = fma(ps, r2, 1.0 / 362880.0);
-    ps = fma(ps, r2, -1.0 / 5040.0);
-    ps = fma(ps, r2, 1.0 / 120.0);
-    ps = fma(ps, r2, -1.0 / 6.0);
-    const float sn = (float)fma(r * r2, ps, r);
-    double pc = -1.0 / 87178291200.0;
-    pc = fma(pc, r2, 1.0 / 479001600.0);
-    pc = fma(pc, r2, -1.0 / 3628800.0);
-    pc = fma(pc, r2, 1.0 / 40320.0);
-    pc = fma(pc, r2, -1.0 / 720.0);
-    pc = fma(pc, r2, 1.0 / 24.0);
+    double ps = fma(r2, -1.0 / 5040, 1.0 / 120);
+    ps = fma(ps, r2, -1.0 / 6);
+    const double sr = fma(r * r2, ps, r);
+    double pc = fma(r2, -1.0 / 720, 1.0 / 24);
     pc = fma(pc, r2, -0.5);
-    const float cs = (float)fma(r2, pc, 1.0);
-    const int qi = (int)q & 3;
-    const float a = (qi & 1) ? cs : sn, b = (qi & 1) ? sn : cs;
-    *s = (qi & 2) ? -a : a;
-    *c = ((qi + 1) & 2) ? -b : b;
+    const double cr = fma(r2, pc, 1.0);
+    *s = (float)fma(SCTAB[k][0], cr, SCTAB[k][1] * sr);
+    *c = (float)fma(SCTAB[k][1], cr, -(SCTAB[k][0] * sr));
 }
 
 static float expf_acc(float xf)

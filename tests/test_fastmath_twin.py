@@ -2,7 +2,8 @@
 (rocrand_amd/csrc/rr_dist.hpp: logf_tab, sincosf_acc, expf_acc) evaluated by a host twin
 (tests/fastmath_twin.c, same double operations, same tables from rr_mathtab.hpp) against
 correctly rounded results on a strided sample of their domains.  Exhaustive runs of the
-same twin (stride 1) give 0 / 1 / 0 / 2 mismatches for log / sin / cos / exp."""
+same twin (stride 1) give 1 / 0 / 0 / 2 mismatches for log / sin / cos / exp, each
+within 2^-48 of a rounding boundary."""
 import os
 import re
 import shutil
@@ -19,12 +20,15 @@ def twin_header(text):
     consts = dict(re.findall(r"constexpr \w+\s+(RR_\w+)\s*=\s*([^;]+);", text))
     log_body = text[text.index("rr_logtab["):text.index("};", text.index("rr_logtab["))]
     exp_body = text[text.index("rr_exptab["):text.index("};", text.index("rr_exptab["))]
+    sc_body = text[text.index("rr_sctab["):text.index("};", text.index("rr_sctab["))]
     logs = re.findall(r"\{(0x[^,]+),\s*([^}]+)\}", log_body)
     exps = re.findall(r"(0x[0-9a-fA-Fp.+-]+),", exp_body)
-    assert len(logs) == 64 and len(exps) == 32
+    scs = re.findall(r"\{(-?0x[^,]+),\s*([^}]+)\}", sc_body)
+    assert len(logs) == 64 and len(exps) == 32 and len(scs) == 64
     out = [f"#define {k[3:]} {v.rstrip('u')}" for k, v in consts.items()]
     out.append("static const double LOGTAB[64][2] = {" + ",".join(f"{{{a},{b}}}" for a, b in logs) + "};")
     out.append("static const double EXPTAB[32] = {" + ",".join(exps) + "};")
+    out.append("static const double SCTAB[64][2] = {" + ",".join(f"{{{a},{b}}}" for a, b in scs) + "};")
     return "\n".join(out) + "\n"
 
 
