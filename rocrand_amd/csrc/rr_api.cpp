@@ -149,6 +149,30 @@ poisson_cache::~poisson_cache()
     (void)hipHostFree(m_hpacked);
 }
 
+rocrand_status poisson_cache::reserve()
+{
+    if(m_dprob)
+        return ROCRAND_STATUS_SUCCESS;
+    if(hipMalloc(&m_dprob, POISSON_MAX_SIZE * sizeof(double)) != hipSuccess
+       || hipMalloc(&m_dalias, POISSON_MAX_SIZE * sizeof(unsigned int)) != hipSuccess
+       || hipHostMalloc(&m_hprob, POISSON_MAX_SIZE * sizeof(double)) != hipSuccess
+       || hipHostMalloc(&m_halias, POISSON_MAX_SIZE * sizeof(unsigned int)) != hipSuccess
+       || hipMalloc(&m_dpacked, POISSON_MAX_SIZE * sizeof(uint64_t)) != hipSuccess
+       || hipHostMalloc(&m_hpacked, POISSON_MAX_SIZE * sizeof(uint64_t)) != hipSuccess
+       || hipEventCreateWithFlags(&m_copied, hipEventDisableTiming) != hipSuccess)
+        return ROCRAND_STATUS_ALLOCATION_FAILED;
+    return ROCRAND_STATUS_SUCCESS;
+}
+
+void poisson_cache::fill_staging(void* arg)
+{
+    const captured_table& t = *static_cast<const captured_table*>(arg);
+    memcpy(t.cache->m_hprob, t.prob.data(), t.size * sizeof(double));
+    memcpy(t.cache->m_halias, t.alias.data(), t.size * sizeof(unsigned int));
+    if(!t.packed.empty())
+        memcpy(t.cache->m_hpacked, t.packed.data(), t.size * sizeof(uint64_t));
+}
+
 rocrand_status poisson_cache::get(double lambda, hipStream_t stream, dist_params& p)
 {
     p.lambda      = lambda;
@@ -156,17 +180,9 @@ rocrand_status poisson_cache::get(double lambda, hipStream_t stream, dist_params
     p.huge        = lambda > LAMBDA_HUGE ? 1 : 0;
     if(p.huge)
         return ROCRAND_STATUS_SUCCESS;
-    if(!m_dprob)
-    {
-        if(hipMalloc(&m_dprob, POISSON_MAX_SIZE * sizeof(double)) != hipSuccess
-           || hipMalloc(&m_dalias, POISSON_MAX_SIZE * sizeof(unsigned int)) != hipSuccess
-           || hipHostMalloc(&m_hprob, POISSON_MAX_SIZE * sizeof(double)) != hipSuccess
-           || hipHostMalloc(&m_halias, POISSON_MAX_SIZE * sizeof(unsigned int)) != hipSuccess
-           || hipMalloc(&m_dpacked, POISSON_MAX_SIZE * sizeof(uint64_t)) != hipSuccess
-           || hipHostMalloc(&m_hpacked, POISSON_MAX_SIZE * sizeof(uint64_t)) != hipSuccess
-           || hipEventCreateWithFlags(&m_copied, hipEventDisableTiming) != hipSuccess)
-            return ROCRAND_STATUS_ALLOCATION_FAILED;
-    }
+    rocrand_status st = reserve();
+    if(st != ROCRAND_STATUS_SUCCESS)
+        return st;
     if(lambda != m_lambda)
     {
         std::vector<double>       prob;
@@ -180,23 +196,43 @@ rocrand_status poisson_cache::get(double lambda, hipStream_t stream, dist_params
             m_size = poisson_table(lambda, m_offset, prob, alias);
         if(m_size > POISSON_MAX_SIZE)
             return ROCRAND_STATUS_INTERNAL_ERROR;
-        // The staging buffers may still feed the previous table's copy.
-        if(hipEventSynchronize(m_copied) != hipSuccess)
-            return ROCRAND_STATUS_INTERNAL_ERROR;
-        memcpy(m_hprob, prob.data(), m_size * sizeof(double));
-        memcpy(m_halias, alias.data(), m_size * sizeof(unsigned int));
+        std::vector<uint64_t> packed;
         if(!cdf_method)
+            packed = pack_alias(prob, alias);
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        if(hipStreamIsCapturing(stream, &cap) != hipSuccess)
+            return ROCRAND_STATUS_INTERNAL_ERROR;
+        if(cap == hipStreamCaptureStatusActive)
         {
-            const std::vector<uint64_t> packed = pack_alias(prob, alias);
-            memcpy(m_hpacked, packed.data(), m_size * sizeof(uint64_t));
+            // the staging is filled when the graph runs, in stream order before the copies
+            auto t    = std::make_unique<captured_table>();
+            t->cache  = this;
+            t->size   = m_size;
+            t->prob   = std::move(prob);
+            t->alias  = std::move(alias);
+            t->packed = std::move(packed);
+            if(hipLaunchHostFunc(stream, &poisson_cache::fill_staging, t.get()) != hipSuccess)
+                return ROCRAND_STATUS_INTERNAL_ERROR;
+            m_captured.push_back(std::move(t));
+        }
+        else
+        {
+            // The staging buffers may still feed the previous table's copy.
+            if(hipEventSynchronize(m_copied) != hipSuccess)
+                return ROCRAND_STATUS_INTERNAL_ERROR;
+            memcpy(m_hprob, prob.data(), m_size * sizeof(double));
+            memcpy(m_halias, alias.data(), m_size * sizeof(unsigned int));
+            if(!cdf_method)
+                memcpy(m_hpacked, packed.data(), m_size * sizeof(uint64_t));
         }
         if(hipMemcpyAsync(m_dprob, m_hprob, m_size * sizeof(double), hipMemcpyHostToDevice, stream) != hipSuccess
            || hipMemcpyAsync(m_dalias, m_halias, m_size * sizeof(unsigned int), hipMemcpyHostToDevice, stream)
                   != hipSuccess
            || (!cdf_method
                && hipMemcpyAsync(m_dpacked, m_hpacked, m_size * sizeof(uint64_t), hipMemcpyHostToDevice, stream)
-                      != hipSuccess)
-           || hipEventRecord(m_copied, stream) != hipSuccess)
+                      != hipSuccess))
+            return ROCRAND_STATUS_INTERNAL_ERROR;
+        if(cap != hipStreamCaptureStatusActive && hipEventRecord(m_copied, stream) != hipSuccess)
             return ROCRAND_STATUS_INTERNAL_ERROR;
         m_lambda = lambda;
     }
@@ -933,7 +969,9 @@ rocrand_status ROCRANDAPI rocrand_generate_poisson(rocrand_generator g, unsigned
 
 rocrand_status ROCRANDAPI rocrand_initialize_generator(rocrand_generator g)
 {
-    return guarded(g, [&] { return g->init(); });
+    // Everything a later generate call would allocate is allocated here, so that calls
+    // captured into a HIP graph allocate nothing (test_rocrand_hipgraphs.cpp:13-66).
+    return guarded(g, [&] { return g->initialize(); });
 }
 rocrand_status ROCRANDAPI rocrand_set_stream(rocrand_generator g, hipStream_t stream)
 {

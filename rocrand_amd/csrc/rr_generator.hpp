@@ -40,7 +40,13 @@ class poisson_cache
 {
 public:
     ~poisson_cache();
-    // Fills p for lambda (<= 4000: table method, else the normal approximation).
+    // Device tables and pinned staging (allocated by rocrand_initialize_generator, so
+    // that generate calls under stream capture allocate nothing).
+    rocrand_status reserve();
+    // Fills p for lambda (<= 4000: table method, else the normal approximation).  A new
+    // lambda's table is staged and copied on `stream`; under stream capture the staging
+    // is filled by a host-function node, so every replay of the graph uploads that
+    // graph's table (poisson.hpp:296-340 does the same with hipLaunchHostFunc).
     rocrand_status get(double lambda, hipStream_t stream, dist_params& p);
     // Quasi-random generators sample by binary search in the cumulative table
     // (sobol.hpp:500-503 selects DISCRETE_METHOD_CDF) instead of the alias table.
@@ -56,6 +62,18 @@ private:
     uint64_t*     m_dpacked = nullptr; // packed alias table (alias method)
     uint64_t*     m_hpacked = nullptr;
     hipEvent_t    m_copied = nullptr;
+    // Tables of captured lambda changes, read by their graphs' host-function nodes; kept
+    // for the generator's lifetime (a graph must not outlive its generator).
+    struct captured_table
+    {
+        poisson_cache*            cache;
+        unsigned int              size;
+        std::vector<double>       prob;
+        std::vector<unsigned int> alias;
+        std::vector<uint64_t>     packed;
+    };
+    std::vector<std::unique_ptr<captured_table>> m_captured;
+    static void fill_staging(void* table);
 };
 
 // Poisson pmf of poisson.hpp:110-161 (unnormalised, truncated where p < 1e-12).
@@ -97,6 +115,16 @@ struct rocrand_generator_base_type
 
     // Shape of kind k for this engine, or TYPE_ERROR.
     virtual rocrand_status shape(rr::out_kind k, rr::dist_shape& s) const = 0;
+
+    // rocrand_initialize_generator: engine set-up plus the allocations a generate call
+    // could otherwise make (none may happen under stream capture).
+    rocrand_status initialize()
+    {
+        rocrand_status st = init();
+        if(st == ROCRAND_STATUS_SUCCESS && host_mode == 0)
+            st = m_poisson.reserve();
+        return st;
+    }
 
 protected:
     virtual rocrand_status generate_device(rr::out_kind k, void* data, size_t n, const rr::gen_args& args) = 0;

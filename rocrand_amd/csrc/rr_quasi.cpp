@@ -191,7 +191,11 @@ public:
     {
         if(m_initialized)
             return ROCRAND_STATUS_SUCCESS;
-        m_current = (uint32_t)offset; // the point index is an unsigned int
+        // the device copy of the direction vectors is made here, outside any capture
+        sobol_device_tables t;
+        if(get_sobol_device_tables(t) != hipSuccess)
+            return ROCRAND_STATUS_ALLOCATION_FAILED;
+        m_current     = (uint32_t)offset; // the point index is an unsigned int
         m_initialized = true;
         return ROCRAND_STATUS_SUCCESS;
     }
