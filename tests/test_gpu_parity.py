@@ -129,6 +129,37 @@ def test_poisson_lambdas(ename, lam):
             _check("poisson", g, want)
 
 
+@pytest.mark.parametrize("ename", ["philox", "mrg32k3a", "xorwow", "lfsr113", "threefry4x32"])
+@pytest.mark.parametrize("nonblocking", [False, True])
+def test_poisson_multiple_lambdas_back_to_back(ename, nonblocking):
+    """20 Poisson calls with lambdas in [0.001, 5000] issued without host waits (the
+    reference's test_rocrand_generate_poisson.cpp:75-137), each against the oracle's
+    continuing stream: table updates between calls are ordered with the kernels."""
+    rng = ENGINES[ename]
+    rs = np.random.default_rng(654321)
+    lams = [float(x) for x in rs.uniform(0.001, 5000.0, 20)]
+    n = 125638
+    stream = torch.cuda.Stream() if nonblocking else None
+    g = R.PRNG(rng, seed=4242)
+    if stream is not None:
+        g.stream = stream
+    bufs = [torch.zeros(n, dtype=torch.uint32, device="cuda") for _ in lams]
+    torch.cuda.synchronize()
+    for b, lam in zip(bufs, lams):
+        g.poisson(b, lam)
+    torch.cuda.synchronize()
+    og = O.OracleGen(rng, seed=4242)
+    for b, lam in zip(bufs, lams):
+        st, want = og.generate("poisson", n, b.data_ptr(), lam, 0.0)
+        assert st == 0
+        got = b.cpu().numpy()
+        if lam > 4000.0:
+            assert np.max(np.abs(got.astype(np.int64) - want.astype(np.int64))) <= 1
+            assert np.mean(got != want) < 1e-3
+        else:
+            assert np.array_equal(got, want), f"lambda {lam}"
+
+
 def test_headline_config1():
     """Philox4x32-10, seed 0x0123456789ABCDEF: 2^24 u32 then 2^24 uniform f32 on one generator."""
     n = 1 << 24
