@@ -1,6 +1,6 @@
 /* Host twin of the float transcendentals of rocrand_amd/csrc/rr_dist.hpp (logf_tab,
  * sincosf_acc, expf_acc): the same double operations in the same order, with the tables
- * of rr_mathtab.hpp (written to mathtab_twin.h by the test).  Compiled with gcc
+ * of rocrand_amd_mathtab.h (written to mathtab_twin.h by the test).  Compiled with gcc
  * -ffp-contract=off; fma() is the correctly rounded fused multiply-add, as the device's
  * v_fma_f64.
  *

@@ -1,6 +1,6 @@
 """The float transcendentals of the Box-Muller and log-normal transforms
 (rocrand_amd/csrc/rr_dist.hpp: logf_tab, sincosf_acc, expf_acc) evaluated by a host twin
-(tests/fastmath_twin.c, same double operations, same tables from rr_mathtab.hpp) against
+(tests/fastmath_twin.c, same double operations, same tables from rocrand_amd_mathtab.h) against
 correctly rounded results on a strided sample of their domains.  Exhaustive runs of the
 same twin (stride 1) give 1 / 0 / 0 / 2 mismatches for log / sin / cos / exp, each
 within 2^-48 of a rounding boundary."""
@@ -12,7 +12,7 @@ import subprocess
 import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-TAB = os.path.join(os.path.dirname(HERE), "rocrand_amd", "csrc", "rr_mathtab.hpp")
+TAB = os.path.join(os.path.dirname(HERE), "include", "rocrand", "rocrand_amd_mathtab.h")
 MAKE = os.path.join(os.path.dirname(HERE), "rocrand_amd", "data", "make_math_tables.py")
 
 
@@ -52,7 +52,7 @@ def test_tables_match_generator(tmp_path):
     spec = importlib.util.spec_from_file_location("make_math_tables", MAKE)
     mod = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(mod)
-    mod.OUT = str(tmp_path / "rr_mathtab.hpp")
+    mod.OUT = str(tmp_path / "rocrand_amd_mathtab.h")
     mod.main()
     with open(TAB) as a, open(mod.OUT) as b:
         assert a.read() == b.read()
