@@ -368,6 +368,47 @@ __host__ __device__ __forceinline__ uint32_t mrg32k3a_next(mrg_state& s)
     return (p1 - p2) + (p1 <= p2 ? (uint32_t)MRG32_M1 : 0u);
 }
 
+// Seeded state words {g1[0..2], g2[0..2]} (rocrand_mrg32k3a.h:103-120, :318-363): the
+// seeding products wrap at 64 bits and reduce with single folds, reproduced literally.
+__host__ __device__ __forceinline__ uint64_t mrg32_seed_mod(uint64_t p, bool m2)
+{
+    const uint64_t c = m2 ? 22853ull : 209ull;
+    const uint64_t m = m2 ? MRG32_M2 : MRG32_M1;
+    p                = c * (p >> 32) + (p & 0xffffffffull);
+    if(m2)
+        p = c * (p >> 32) + (p & 0xffffffffull);
+    return p >= m ? p - m : p;
+}
+__host__ __device__ __forceinline__ uint64_t mrg32_seed_mul(uint32_t i, uint64_t j, bool m2)
+{
+    const uint64_t hi = i / 131072u;
+    const uint64_t lo = i - hi * 131072u;
+    return mrg32_seed_mod(mrg32_seed_mod(hi * j, m2) * 131072u + mrg32_seed_mod(lo * j, m2), m2);
+}
+__host__ __device__ __forceinline__ void mrg32k3a_seed_words(uint64_t s, uint32_t w[6])
+{
+    const uint32_t x = (uint32_t)s ^ 0x55555555u;
+    const uint32_t y = (uint32_t)((s >> 32) ^ 0xAAAAAAAAu);
+    w[0]             = (uint32_t)mrg32_seed_mul(x, s, false);
+    w[1]             = (uint32_t)mrg32_seed_mul(y, s, false);
+    w[2]             = (uint32_t)mrg32_seed_mul(x, s, false);
+    w[3]             = (uint32_t)mrg32_seed_mul(y, s, true);
+    w[4]             = (uint32_t)mrg32_seed_mul(x, s, true);
+    w[5]             = (uint32_t)mrg32_seed_mul(y, s, true);
+}
+// rocrand_mrg31k3p.h seed(): plain remainders of the wrapped 64-bit products.
+__host__ __device__ __forceinline__ void mrg31k3p_seed_words(uint64_t s, uint32_t w[6])
+{
+    const uint32_t x = (uint32_t)(s ^ 0x55555555u);
+    const uint32_t y = (uint32_t)((s >> 32) ^ 0xAAAAAAAAu);
+    w[0]             = (uint32_t)(((uint64_t)x * s) % MRG31_M1);
+    w[1]             = (uint32_t)(((uint64_t)y * s) % MRG31_M1);
+    w[2]             = (uint32_t)(((uint64_t)x * s) % MRG31_M1);
+    w[3]             = (uint32_t)(((uint64_t)y * s) % MRG31_M2);
+    w[4]             = (uint32_t)(((uint64_t)x * s) % MRG31_M2);
+    w[5]             = (uint32_t)(((uint64_t)y * s) % MRG31_M2);
+}
+
 // x - m if x >= m, else x (x < 2^32): the smaller of x and x - m (mod 2^32).
 __host__ __device__ __forceinline__ uint32_t sub_if_ge(uint32_t x, uint32_t m)
 {

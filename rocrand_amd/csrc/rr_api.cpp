@@ -542,32 +542,6 @@ private:
 // (mrg.hpp:159-497, xorwow.hpp:164-471, lfsr113.hpp:175-505)
 // ============================================================================
 
-// rocrand_mrg32k3a.h:318-363: the seeding products wrap at 64 bits and reduce
-// with single folds, reproduced literally.
-uint64_t m32_mod_m1(uint64_t p)
-{
-    p = 209ull * (p >> 32) + (p & 0xffffffffull);
-    if(p >= MRG32_M1)
-        p -= MRG32_M1;
-    return p;
-}
-uint64_t m32_mod_m2(uint64_t p)
-{
-    p = 22853ull * (p >> 32) + (p & 0xffffffffull);
-    p = 22853ull * (p >> 32) + (p & 0xffffffffull);
-    if(p >= MRG32_M2)
-        p -= MRG32_M2;
-    return p;
-}
-uint64_t m32_mod_mul(uint32_t i, uint64_t j, uint64_t (*mod)(uint64_t))
-{
-    const uint64_t hi = i / 131072u;
-    const uint64_t lo = i - hi * 131072u;
-    const uint64_t t1 = mod(hi * j) * 131072u;
-    const uint64_t t2 = mod(lo * j);
-    return mod(t1 + t2);
-}
-
 class interleaved_generator final : public rocrand_generator_base_type
 {
 public:
@@ -658,31 +632,9 @@ public:
     void seed_state(uint32_t* w) const
     {
         if(m_kind == inter_kind::mrg32k3a)
-        {
-            // rocrand_mrg32k3a.h:103-120
-            const uint64_t s = m_seed;
-            const uint32_t x = (uint32_t)s ^ 0x55555555u;
-            const uint32_t y = (uint32_t)((s >> 32) ^ 0xAAAAAAAAu);
-            w[0] = (uint32_t)m32_mod_mul(x, s, m32_mod_m1);
-            w[1] = (uint32_t)m32_mod_mul(y, s, m32_mod_m1);
-            w[2] = (uint32_t)m32_mod_mul(x, s, m32_mod_m1);
-            w[3] = (uint32_t)m32_mod_mul(y, s, m32_mod_m2);
-            w[4] = (uint32_t)m32_mod_mul(x, s, m32_mod_m2);
-            w[5] = (uint32_t)m32_mod_mul(y, s, m32_mod_m2);
-        }
+            mrg32k3a_seed_words(m_seed, w); // rocrand_mrg32k3a.h:103-120
         else if(m_kind == inter_kind::mrg31k3p)
-        {
-            // rocrand_mrg31k3p.h seed(): plain remainders of the wrapped 64-bit products.
-            const uint64_t s = m_seed;
-            const uint32_t x = (uint32_t)(s ^ 0x55555555u);
-            const uint32_t y = (uint32_t)((s >> 32) ^ 0xAAAAAAAAu);
-            w[0] = (uint32_t)(((uint64_t)x * s) % MRG31_M1);
-            w[1] = (uint32_t)(((uint64_t)y * s) % MRG31_M1);
-            w[2] = (uint32_t)(((uint64_t)x * s) % MRG31_M1);
-            w[3] = (uint32_t)(((uint64_t)y * s) % MRG31_M2);
-            w[4] = (uint32_t)(((uint64_t)x * s) % MRG31_M2);
-            w[5] = (uint32_t)(((uint64_t)y * s) % MRG31_M2);
-        }
+            mrg31k3p_seed_words(m_seed, w);
         else if(m_kind == inter_kind::xorwow)
         {
             xorwow_state st;

@@ -21,46 +21,6 @@ namespace rr
 namespace
 {
 
-// rocrand_normal.h:174-197 roc_f_erfinv, float arithmetic rounded step by step.
-__device__ __forceinline__ float roc_f_erfinv(float x)
-{
-    const float sgn = (x < 0.0f) ? -1.0f : 1.0f;
-    x               = __fmul_rn(__fsub_rn(1.0f, x), __fadd_rn(1.0f, x));
-    // (1 - x)(1 + x) is a positive normal float or 0 here (|x| <= 1)
-    const float lnx = x > 0.0f ? logf_tab(x) : logf_acc(x);
-    if(isnan(lnx))
-        return 1.0f;
-    if(isinf(lnx))
-        return 0.0f;
-    constexpr float c1  = 2.0f / (3.141592653f * 0.147f);
-    constexpr float c2  = 1.0f / 0.147f;
-    const float     tt1 = __fadd_rn(c1, __fmul_rn(0.5f, lnx));
-    const float     tt2 = __fmul_rn(c2, lnx);
-    return __fmul_rn(sgn, sqrtf(__fadd_rn(-tt1, sqrtf(__fsub_rn(__fmul_rn(tt1, tt1), tt2)))));
-}
-
-// Uniforms of a direction value (rocrand_uniform.h:63-116 on the 32- or 64-bit value).
-__device__ __forceinline__ float sob_uf32(uint32_t x) { return uniform_f32(x); }
-__device__ __forceinline__ float sob_uf32(uint64_t x) { return uniform_f32(x); }
-__device__ __forceinline__ double sob_uf64(uint32_t x)
-{
-    return __builtin_fma((double)x, R_2POW32_INV_DBL, R_2POW32_INV_DBL);
-}
-__device__ __forceinline__ double sob_uf64(uint64_t x) { return uniform_f64(x); }
-
-// rocrand_normal.h:228-296 normal_distribution(_double) on a direction value.
-template<class V>
-__device__ __forceinline__ float sob_normal_f32(V x)
-{
-    const float p = sob_uf32(x);
-    return __fmul_rn(R_SQRT2, roc_f_erfinv(__fsub_rn(__fmul_rn(2.0f, p), 1.0f)));
-}
-template<class V>
-__device__ __forceinline__ double sob_normal_f64(V x)
-{
-    return normal_double_1(sob_uf64(x));
-}
-
 // rocrand_discrete.h:109-165: binary search of the CDF; the draw is x * 2^-32
 // (32-bit values) or x * 2^-64 (64-bit values).
 __device__ __forceinline__ unsigned int discrete_cdf(double x, const alias_table& t)
