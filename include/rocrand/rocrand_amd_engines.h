@@ -12,6 +12,11 @@
 
 #include <stdint.h>
 
+// Floating-point expressions in these headers round every operation separately (no
+// fused multiply-add contraction), whatever -ffp-contract the including code uses; the
+// pragma applies to the function body it opens.
+#define ROCRAND_AMD_FP_EXACT _Pragma("clang fp contract(off)")
+
 namespace rocrand_amd_engine
 {
 

@@ -16,8 +16,6 @@
 
 #include <stdint.h>
 
-#pragma float_control(push)
-#pragma clang fp contract(off)
 
 namespace rocrand_amd_engine
 {
@@ -60,6 +58,7 @@ constexpr double RR_LN2     = 6.93147180559945286227e-01; // ln 2 rounded to dou
 // ln u for a positive normal float u (the Box-Muller uniforms lie in [2^-32, 1]).
 __device__ __forceinline__ float logf_tab(float u)
 {
+    ROCRAND_AMD_FP_EXACT
     const uint32_t ix  = __float_as_uint(u);
     const uint32_t tmp = ix - RR_LOGTAB_OFF;
     const uint32_t i   = (tmp >> (23 - RR_LOGTAB_BITS)) & ((1u << RR_LOGTAB_BITS) - 1u);
@@ -78,6 +77,7 @@ __device__ __forceinline__ float logf_tab(float u)
 
 __device__ __forceinline__ double log_dbl(double x)
 {
+    ROCRAND_AMD_FP_EXACT
     int    e = __builtin_amdgcn_frexp_exp(x);
     double m = __builtin_amdgcn_frexp_mant(x); // [0.5, 1)
     if(m < 0.70710678118654752440)
@@ -116,6 +116,7 @@ __device__ __forceinline__ double log_dbl(double x)
 // sin x or cos x is near 0 the table entry is an exact 0 / +-1, so the sum does not cancel.
 __device__ __forceinline__ void sincosf_acc(float xf, float* s, float* c)
 {
+    ROCRAND_AMD_FP_EXACT
     const double  x  = (double)xf;
     const double  q  = __builtin_rint(x * RR_SC_INV_STEP);
     const int     k  = (int)q & ((1 << RR_SCTAB_BITS) - 1);
@@ -139,6 +140,7 @@ __device__ __forceinline__ void sincosf_acc(float xf, float* s, float* c)
 // (below 2^-96) and of infinities, which the Box-Muller radius (0 or >= 2^-24) never is.
 __device__ __forceinline__ float sqrtf_radius(float x)
 {
+    ROCRAND_AMD_FP_EXACT
     const float s  = __builtin_amdgcn_sqrtf(x);
     const float sm = __uint_as_float(__float_as_uint(s) - 1u);
     const float sp = __uint_as_float(__float_as_uint(s) + 1u);
@@ -154,6 +156,7 @@ __device__ __forceinline__ float logf_acc(float x) { return (float)log_dbl((doub
 // first (the float result is then 0 or +inf already), which keeps |32 e + j| < 2^13.
 __device__ __forceinline__ float expf_acc(float xf)
 {
+    ROCRAND_AMD_FP_EXACT
     const double x  = (double)xf;
     const double xc = x < -110.0 ? -110.0 : (x > 100.0 ? 100.0 : x);
     const double kd = __builtin_rint(xc * RR_EXP_INV_STEP);
@@ -217,104 +220,115 @@ struct mrg_consts<31>
 template<int MRG>
 __device__ __forceinline__ uint32_t mrg_uint(uint32_t v)
 {
-    return (uint32_t)__dmul_rn((double)(v - 1), mrg_consts<MRG>::uint_norm);
+    ROCRAND_AMD_FP_EXACT
+    return (uint32_t)((double)(v - 1) * mrg_consts<MRG>::uint_norm);
 }
 template<int MRG>
 __device__ __forceinline__ float mrg_f32(uint32_t v)
 {
-    return (float)__dmul_rn((double)v, mrg_consts<MRG>::norm);
+    ROCRAND_AMD_FP_EXACT
+    return (float)((double)v * mrg_consts<MRG>::norm);
 }
 template<int MRG>
 __device__ __forceinline__ double mrg_f64(uint32_t v)
 {
-    return __dmul_rn((double)v, mrg_consts<MRG>::norm);
+    ROCRAND_AMD_FP_EXACT
+    return (double)v * mrg_consts<MRG>::norm;
 }
 
 // rocrand_normal.h:51-66 (host branch: separate sinf/cosf of the same angle).
 __device__ __forceinline__ void box_muller(uint32_t x, uint32_t y, float& r0, float& r1)
 {
+    ROCRAND_AMD_FP_EXACT
     const float u = uniform_f32(x);
-    const float v = __fadd_rn(R_2POW32_INV_2PI, __fmul_rn((float)y, R_2POW32_INV_2PI));
-    const float s = sqrtf_radius(__fmul_rn(-2.0f, logf_tab(u)));
+    const float v = R_2POW32_INV_2PI + (float)y * R_2POW32_INV_2PI;
+    const float s = sqrtf_radius(-2.0f * logf_tab(u));
     float       sn, cs;
     sincosf_acc(v, &sn, &cs);
-    r0 = __fmul_rn(sn, s);
-    r1 = __fmul_rn(cs, s);
+    r0 = sn * s;
+    r1 = cs * s;
 }
 // rocrand_normal.h:76-96 (host branch: sin(w * pi), not sincospi).
 __device__ __forceinline__ void box_muller_double(uint32_t a, uint32_t b, uint32_t c, uint32_t d, double& r0,
                                                   double& r1)
 {
+    ROCRAND_AMD_FP_EXACT
     const uint64_t v1 = (uint64_t)a ^ ((uint64_t)b << 21);
-    const double   u  = __dadd_rn(R_2POW53_INV_DBL, __dmul_rn((double)v1, R_2POW53_INV_DBL));
+    const double   u  = R_2POW53_INV_DBL + (double)v1 * R_2POW53_INV_DBL;
     const uint64_t v2 = (uint64_t)c ^ ((uint64_t)d << 21);
-    const double   w  = __dadd_rn(R_2POW53_INV_DBL * 2.0, __dmul_rn((double)v2, R_2POW53_INV_DBL * 2.0));
-    const double   s  = sqrt(__dmul_rn(-2.0, log(u)));
-    const double   t  = __dmul_rn(w, R_PI_DBL);
+    const double   w  = R_2POW53_INV_DBL * 2.0 + (double)v2 * (R_2POW53_INV_DBL * 2.0);
+    const double   s  = sqrt(-2.0 * log(u));
+    const double   t  = w * R_PI_DBL;
     double         sn, cs;
     sincos(t, &sn, &cs);
-    r0 = __dmul_rn(sn, s);
-    r1 = __dmul_rn(cs, s);
+    r0 = sn * s;
+    r1 = cs * s;
 }
 // rocrand_normal.h:108-135, host (float math) branch.
 __device__ __forceinline__ void box_muller_half(uint32_t x16, uint32_t y16, float& h0, float& h1)
 {
-    const float u = __fadd_rn(R_2POW16_INV, __fmul_rn((float)(x16 & 0xffffu), R_2POW16_INV));
-    const float v = __fadd_rn(R_2POW16_INV_2PI, __fmul_rn((float)(y16 & 0xffffu), R_2POW16_INV_2PI));
-    const float s = sqrtf_radius(__fmul_rn(-2.0f, logf_tab(u)));
+    ROCRAND_AMD_FP_EXACT
+    const float u = R_2POW16_INV + (float)(x16 & 0xffffu) * R_2POW16_INV;
+    const float v = R_2POW16_INV_2PI + (float)(y16 & 0xffffu) * R_2POW16_INV_2PI;
+    const float s = sqrtf_radius(-2.0f * logf_tab(u));
     float       sn, cs;
     sincosf_acc(v, &sn, &cs);
-    h0 = __half2float(__float2half(__fmul_rn(sn, s)));
-    h1 = __half2float(__float2half(__fmul_rn(cs, s)));
+    h0 = __half2float(__float2half(sn * s));
+    h1 = __half2float(__float2half(cs * s));
 }
 // rocrand_normal.h:138-154
 template<int MRG>
 __device__ __forceinline__ void mrg_box_muller(uint32_t x, uint32_t y, float& r0, float& r1)
 {
+    ROCRAND_AMD_FP_EXACT
     const float u = mrg_f32<MRG>(x);
-    const float v = __fmul_rn(mrg_f32<MRG>(y), R_2PI);
-    const float s = sqrtf_radius(__fmul_rn(-2.0f, logf_tab(u)));
+    const float v = mrg_f32<MRG>(y) * R_2PI;
+    const float s = sqrtf_radius(-2.0f * logf_tab(u));
     float       sn, cs;
     sincosf_acc(v, &sn, &cs);
-    r0 = __fmul_rn(sn, s);
-    r1 = __fmul_rn(cs, s);
+    r0 = sn * s;
+    r1 = cs * s;
 }
 // rocrand_normal.h:156-172: float-precision uniforms widened to double.
 template<int MRG>
 __device__ __forceinline__ void mrg_box_muller_double(uint32_t x, uint32_t y, double& r0, double& r1)
 {
+    ROCRAND_AMD_FP_EXACT
     const double u = (double)mrg_f32<MRG>(x);
-    const double v = __dmul_rn((double)mrg_f32<MRG>(y), 2.0);
-    const double s = sqrt(__dmul_rn(-2.0, log(u)));
+    const double v = (double)mrg_f32<MRG>(y) * 2.0;
+    const double s = sqrt(-2.0 * log(u));
     double       sn, cs;
-    sincos(__dmul_rn(v, R_PI_DBL), &sn, &cs);
-    r0 = __dmul_rn(sn, s);
-    r1 = __dmul_rn(cs, s);
+    sincos(v * R_PI_DBL, &sn, &cs);
+    r0 = sn * s;
+    r1 = cs * s;
 }
 // rocrand_normal.h:199-221 roc_d_erfinv and :263-268 normal_distribution_double.
 __device__ __forceinline__ double roc_d_erfinv(double x)
 {
+    ROCRAND_AMD_FP_EXACT
     const double sgn = (x < 0.0) ? -1.0 : 1.0;
-    x                = __dmul_rn(__dsub_rn(1.0, x), __dadd_rn(1.0, x));
+    x                = (1.0 - x) * (1.0 + x);
     const double lnx = log(x);
     if(isnan(lnx))
         return 1.0;
     if(isinf(lnx))
         return 0.0;
-    const double tt1 = __dadd_rn(2.0 / (R_PI_DBL * 0.147), __dmul_rn(0.5, lnx));
-    const double tt2 = __dmul_rn(1.0 / 0.147, lnx);
-    return __dmul_rn(sgn, sqrt(__dadd_rn(-tt1, sqrt(__dsub_rn(__dmul_rn(tt1, tt1), tt2)))));
+    const double tt1 = 2.0 / (R_PI_DBL * 0.147) + 0.5 * lnx;
+    const double tt2 = 1.0 / 0.147 * lnx;
+    return sgn * sqrt(-tt1 + sqrt(tt1 * tt1 - tt2));
 }
 __device__ __forceinline__ double normal_double_1(double p)
 {
-    return __dmul_rn((double)R_SQRT2, roc_d_erfinv(__dsub_rn(__dmul_rn(2.0, p), 1.0)));
+    ROCRAND_AMD_FP_EXACT
+    return (double)R_SQRT2 * roc_d_erfinv(2.0 * p - 1.0);
 }
 
 // rocrand_normal.h:174-197 roc_f_erfinv, float arithmetic rounded step by step.
 __device__ __forceinline__ float roc_f_erfinv(float x)
 {
+    ROCRAND_AMD_FP_EXACT
     const float sgn = (x < 0.0f) ? -1.0f : 1.0f;
-    x               = __fmul_rn(__fsub_rn(1.0f, x), __fadd_rn(1.0f, x));
+    x               = (1.0f - x) * (1.0f + x);
     // (1 - x)(1 + x) is a positive normal float or 0 here (|x| <= 1)
     const float lnx = x > 0.0f ? logf_tab(x) : logf_acc(x);
     if(isnan(lnx))
@@ -323,9 +337,9 @@ __device__ __forceinline__ float roc_f_erfinv(float x)
         return 0.0f;
     constexpr float c1  = 2.0f / (3.141592653f * 0.147f);
     constexpr float c2  = 1.0f / 0.147f;
-    const float     tt1 = __fadd_rn(c1, __fmul_rn(0.5f, lnx));
-    const float     tt2 = __fmul_rn(c2, lnx);
-    return __fmul_rn(sgn, sqrtf(__fadd_rn(-tt1, sqrtf(__fsub_rn(__fmul_rn(tt1, tt1), tt2)))));
+    const float     tt1 = c1 + 0.5f * lnx;
+    const float     tt2 = c2 * lnx;
+    return sgn * sqrtf(-tt1 + sqrtf(tt1 * tt1 - tt2));
 }
 
 // Uniforms of a direction value (rocrand_uniform.h:63-116 on the 32- or 64-bit value).
@@ -341,8 +355,9 @@ __device__ __forceinline__ double sob_uf64(uint64_t x) { return uniform_f64(x); 
 template<class V>
 __device__ __forceinline__ float sob_normal_f32(V x)
 {
+    ROCRAND_AMD_FP_EXACT
     const float p = sob_uf32(x);
-    return __fmul_rn(R_SQRT2, roc_f_erfinv(__fsub_rn(__fmul_rn(2.0f, p), 1.0f)));
+    return R_SQRT2 * roc_f_erfinv(2.0f * p - 1.0f);
 }
 template<class V>
 __device__ __forceinline__ double sob_normal_f64(V x)
@@ -352,4 +367,3 @@ __device__ __forceinline__ double sob_normal_f64(V x)
 
 } // namespace rocrand_amd_engine
 
-#pragma float_control(pop)

@@ -1,0 +1,33 @@
+/* The reference's device API (library/include/rocrand/rocrand_kernel.h), compiled
+ * host-only from the reference tree and driven by tests/devapi/program.h: its host
+ * branch run over the same per-thread call sequences as the GPU parity tests.  Linked
+ * into oracle/_ref/libref.so by build_ref.sh.  Test infrastructure only.
+ *
+ * As in ref_harness.cpp, __HIP_PLATFORM_AMD__ is undefined once the HIP headers are in:
+ * it selects `result.data[...]`, a member current HIP vector types no longer have, over
+ * the reference's portable `(&result.x)[...]` branch (the same arithmetic). */
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#undef __HIP_PLATFORM_AMD__
+
+#include <rocrand/rocrand_kernel.h>
+
+#include "../tests/devapi/program.h"
+
+extern "C" int ref_devapi_run(int engine, int fn, unsigned long long seed, const unsigned long long* sub,
+                              const unsigned long long* off, int n, int calls, double a, double b,
+                              unsigned int dsize, unsigned int doffset, const double* prob,
+                              const unsigned int* alias, unsigned long long* out)
+{
+    if(n < 0 || calls < 0 || ((fn == devapi_test::DISCRETE || fn == devapi_test::DISCRETE4) && dsize == 0))
+        return -2;
+    rocrand_discrete_distribution_st d;
+    d.size        = dsize;
+    d.offset      = doffset;
+    d.alias       = const_cast<unsigned int*>(alias);
+    d.probability = const_cast<double*>(prob);
+    d.cdf         = nullptr;
+    return devapi_test::run_host(engine, fn, seed, sub, off, n, calls, a, b, &d, (uint64_t*)out);
+}

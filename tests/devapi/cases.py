@@ -1,0 +1,112 @@
+"""Device-API parity cases shared by the fixture generator and the tests: engine and
+function ids of tests/devapi/program.h, the per-thread (subsequence, offset) pairs, the
+Poisson lambdas and the discrete table, and ctypes bindings of the three drivers
+(reference host build, our host branch, our GPU kernels)."""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+HOST_LIB = os.path.join(HERE, "libdevapi_host.so")
+GPU_LIB = os.path.join(HERE, "libdevapi_gpu.so")
+
+ENGINES = ["philox", "mrg32k3a", "mrg31k3p", "xorwow", "lfsr113", "threefry2x32", "threefry2x64",
+           "threefry4x32", "threefry4x64"]
+FNS = ["WORD", "WORDV", "UNIFORM", "UNIFORM2", "UNIFORM4", "UNIFORM_D", "UNIFORM_D2", "UNIFORM_D4", "NORMAL",
+       "NORMAL2", "NORMAL4", "NORMAL_D", "NORMAL_D2", "NORMAL_D4", "LOGN", "LOGN2", "LOGN4", "LOGN_D", "LOGN_D2",
+       "LOGN_D4", "POISSON", "POISSON4", "DISCRETE", "DISCRETE4", "MIXED", "SKIP"]
+E = {n: i for i, n in enumerate(ENGINES)}
+F = {n: i for i, n in enumerate(FNS)}
+# engines whose single normals use the inverse error function and whose Poisson inverts
+ONE_WORD = {"lfsr113", "threefry2x32", "threefry2x64", "threefry4x32", "threefry4x64"}
+WORD64 = {"threefry2x64", "threefry4x64"}
+
+SEED = 0x0123456789ABCDEF
+MEAN, SD = 0.3, 0.7
+LAMBDAS = (0.5, 7.0, 63.9, 64.0, 300.0, 500.0, 999.0, 1000.0, 4000.0, 4001.0, 20000.0)
+# the reference's inversion does not terminate in reasonable time for 500 < lambda < 1000
+REF_ITR_GAP = (500.0, 1000.0)
+
+
+def lambdas_for(engine):
+    if engine in ONE_WORD:
+        return [x for x in LAMBDAS if not REF_ITR_GAP[0] < x < REF_ITR_GAP[1]]
+    return list(LAMBDAS)
+
+
+def thread_params(n, seed=7):
+    """Per-thread subsequences and offsets: edge values first, then random 64-bit ones."""
+    rng = np.random.default_rng(seed)
+    sub = rng.integers(0, 2**64 - 1, n, dtype=np.uint64, endpoint=True)
+    off = rng.integers(0, 2**64 - 1, n, dtype=np.uint64, endpoint=True)
+    edges_sub = [0, 1, 2, 3, 2**32 - 1, 2**32, 2**63, 2**64 - 1]
+    edges_off = [0, 1, 3, 4, 5, 2**32 + 3, 2**62 + 1, 2**64 - 1]
+    k = min(n, len(edges_sub))
+    sub[:k] = np.array(edges_sub[:k], dtype=np.uint64)
+    off[:k] = np.array(edges_off[:k], dtype=np.uint64)
+    return sub, off
+
+
+def discrete_table(size=37, seed=11):
+    """Any probability in [0, 1) and alias below size: parity concerns the sampler only."""
+    rng = np.random.default_rng(seed)
+    return rng.random(size), rng.integers(0, size, size).astype(np.uint32), 5
+
+
+def components(engine, fn):
+    name = FNS[fn]
+    if name == "WORDV":
+        return 2 if engine.startswith("threefry2") else 4
+    if name.endswith("4"):
+        return 4
+    if name.endswith("2"):
+        return 2
+    return 1
+
+
+def _bind(lib, name):
+    f = getattr(lib, name)
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                  ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint, ctypes.c_uint, ctypes.c_void_p,
+                  ctypes.c_void_p, ctypes.c_void_p]
+    return f
+
+
+class Driver:
+    """One of the three drivers; run() returns the [n, calls, 4] uint64 slots or None when
+    the state type has no such call."""
+
+    def __init__(self, path, symbol):
+        self.lib = ctypes.CDLL(path)
+        self.fn = _bind(self.lib, symbol)
+
+    def run(self, engine, fn, sub, off, calls, a=MEAN, b=SD, seed=SEED, table=None):
+        n = len(sub)
+        sub = np.ascontiguousarray(sub, dtype=np.uint64)
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        prob, alias, doff = table if table is not None else discrete_table()
+        prob = np.ascontiguousarray(prob, dtype=np.float64)
+        alias = np.ascontiguousarray(alias, dtype=np.uint32)
+        out = np.zeros(n * calls * 4, dtype=np.uint64)
+        rc = self.fn(E[engine], fn, seed, sub.ctypes.data, off.ctypes.data, n, calls, a, b, len(prob), doff,
+                     prob.ctypes.data, alias.ctypes.data, out.ctypes.data)
+        if rc == -1:
+            return None
+        assert rc == 0, f"driver returned {rc}"
+        return out.reshape(n, calls, 4)
+
+
+def reference():
+    return Driver(REF_LIB, "ref_devapi_run") if os.path.exists(REF_LIB) else None
+
+
+def host():
+    return Driver(HOST_LIB, "devapi_host_run")
+
+
+def gpu():
+    return Driver(GPU_LIB, "devapi_gpu_run")

@@ -1,0 +1,104 @@
+/* Device branch of include/rocrand/rocrand_kernel.h driven by program.h: one GPU thread
+ * per call sequence, outputs copied back to the caller's host buffer. */
+#include <rocrand/rocrand_kernel.h>
+
+#include "program.h"
+
+#include <hip/hip_runtime.h>
+
+namespace
+{
+
+constexpr int BLOCK = 128;
+
+template<class S, int FN>
+__global__ __launch_bounds__(BLOCK) void devapi_kernel(unsigned long long seed, const unsigned long long* sub,
+                                                       const unsigned long long* off, int n, int calls, double a,
+                                                       double b, rocrand_discrete_distribution dist, uint64_t* out)
+{
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if(t >= n)
+        return;
+    devapi_test::run_thread<S, FN>(seed, sub[t], off[t], calls, a, b, dist, out + (size_t)t * calls * 4);
+}
+
+struct device_buffers
+{
+    void* p[6] = {};
+    ~device_buffers()
+    {
+        for(void* q : p)
+            if(q)
+                (void)hipFree(q);
+    }
+};
+
+} // namespace
+
+extern "C" int devapi_gpu_run(int engine, int fn, unsigned long long seed, const unsigned long long* sub,
+                              const unsigned long long* off, int n, int calls, double a, double b,
+                              unsigned int dsize, unsigned int doffset, const double* prob,
+                              const unsigned int* alias, unsigned long long* out)
+{
+    const bool discrete = fn == devapi_test::DISCRETE || fn == devapi_test::DISCRETE4;
+    if(n <= 0 || calls <= 0 || (size_t)n * calls > (1u << 26) || (discrete && (dsize == 0 || !prob || !alias)))
+        return -2;
+    bool known = false;
+    devapi_test::with_engine(engine,
+                             [&](auto* tag)
+                             {
+                                 using S = std::remove_pointer_t<decltype(tag)>;
+                                 known   = devapi_test::supported<S>(fn);
+                             });
+    if(!known)
+        return -1;
+
+    device_buffers     buf;
+    const size_t       out_bytes = (size_t)n * calls * 4 * sizeof(uint64_t);
+    const unsigned int dn        = discrete ? dsize : 1;
+    if(hipMalloc(&buf.p[0], n * sizeof(unsigned long long)) != hipSuccess
+       || hipMalloc(&buf.p[1], n * sizeof(unsigned long long)) != hipSuccess
+       || hipMalloc(&buf.p[2], out_bytes) != hipSuccess || hipMalloc(&buf.p[3], dn * sizeof(double)) != hipSuccess
+       || hipMalloc(&buf.p[4], dn * sizeof(unsigned int)) != hipSuccess
+       || hipMalloc(&buf.p[5], sizeof(rocrand_discrete_distribution_st)) != hipSuccess)
+        return -3;
+    rocrand_discrete_distribution_st d;
+    d.size        = dn;
+    d.offset      = doffset;
+    d.alias       = (unsigned int*)buf.p[4];
+    d.probability = (double*)buf.p[3];
+    d.cdf         = nullptr;
+    if(hipMemcpy(buf.p[0], sub, n * sizeof(unsigned long long), hipMemcpyHostToDevice) != hipSuccess
+       || hipMemcpy(buf.p[1], off, n * sizeof(unsigned long long), hipMemcpyHostToDevice) != hipSuccess
+       || hipMemset(buf.p[2], 0, out_bytes) != hipSuccess
+       || hipMemcpy(buf.p[5], &d, sizeof(d), hipMemcpyHostToDevice) != hipSuccess)
+        return -3;
+    if(discrete
+       && (hipMemcpy(buf.p[3], prob, dn * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
+           || hipMemcpy(buf.p[4], alias, dn * sizeof(unsigned int), hipMemcpyHostToDevice) != hipSuccess))
+        return -3;
+
+    const unsigned int blocks = (unsigned int)((n + BLOCK - 1) / BLOCK);
+    devapi_test::with_engine(
+        engine,
+        [&](auto* tag)
+        {
+            using S = std::remove_pointer_t<decltype(tag)>;
+            devapi_test::with_fn(fn,
+                                 [&](auto fc)
+                                 {
+                                     constexpr int FN = decltype(fc)::value;
+                                     if constexpr(devapi_test::supported<S>(FN))
+                                         hipLaunchKernelGGL((devapi_kernel<S, FN>), dim3(blocks), dim3(BLOCK), 0, 0,
+                                                            seed, (const unsigned long long*)buf.p[0],
+                                                            (const unsigned long long*)buf.p[1], n, calls, a, b,
+                                                            (rocrand_discrete_distribution)buf.p[5],
+                                                            (uint64_t*)buf.p[2]);
+                                 });
+        });
+    if(hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        return -4;
+    if(hipMemcpy(out, buf.p[2], out_bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        return -3;
+    return 0;
+}

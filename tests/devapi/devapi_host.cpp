@@ -1,0 +1,21 @@
+/* Host branch of include/rocrand/rocrand_kernel.h driven by program.h: the same calls as
+ * the GPU kernels of devapi_gpu.hip, run on the CPU with the host libm. */
+#include <rocrand/rocrand_kernel.h>
+
+#include "program.h"
+
+extern "C" int devapi_host_run(int engine, int fn, unsigned long long seed, const unsigned long long* sub,
+                               const unsigned long long* off, int n, int calls, double a, double b,
+                               unsigned int dsize, unsigned int doffset, const double* prob,
+                               const unsigned int* alias, unsigned long long* out)
+{
+    if(n < 0 || calls < 0 || ((fn == devapi_test::DISCRETE || fn == devapi_test::DISCRETE4) && dsize == 0))
+        return -2;
+    rocrand_discrete_distribution_st d;
+    d.size        = dsize;
+    d.offset      = doffset;
+    d.alias       = const_cast<unsigned int*>(alias);
+    d.probability = const_cast<double*>(prob);
+    d.cdf         = nullptr;
+    return devapi_test::run_host(engine, fn, seed, sub, off, n, calls, a, b, &d, (uint64_t*)out);
+}

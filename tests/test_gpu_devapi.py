@@ -1,0 +1,163 @@
+"""Device API (include/rocrand/rocrand_kernel.h) in gfx950 kernels: one GPU thread per call
+sequence of tests/devapi/program.h (tests/devapi/libdevapi_gpu.so).
+
+* Against the reference's own device API: the committed fixtures
+  tests/golden/devapi_golden.npz (every engine and call) and, at 4096 threads, the
+  reference build oracle/_ref/libref.so where it is present (else our host branch, which
+  tests/test_devapi_host.py pins to that build bit for bit).
+* Bars: words, uniforms, Poisson, discrete and skipahead bit-exact; Box-Muller and
+  inverse-error-function normals and log-normals within the bars of parity_checks.py
+  (the device's transcendentals are correctly rounded, the host libm's are not always).
+* Philox4x32-10 and ThreeFry 2x32 / 4x32 states at (seed, 0, offset) yield the bulk
+  generator's output for that seed (rocrand_generate, legacy ordering).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "devapi"))
+import cases as C  # noqa: E402
+from parity_checks import check_box_muller, check_erfinv, check_exact  # noqa: E402
+
+GOLDEN = os.path.join(HERE, "golden", "devapi_golden.npz")
+
+FLOAT_FNS = {"UNIFORM", "UNIFORM2", "UNIFORM4", "NORMAL", "NORMAL2", "NORMAL4", "LOGN", "LOGN2", "LOGN4"}
+DOUBLE_FNS = {"UNIFORM_D", "UNIFORM_D2", "UNIFORM_D4", "NORMAL_D", "NORMAL_D2", "NORMAL_D4", "LOGN_D", "LOGN_D2",
+              "LOGN_D4"}
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    if not os.path.exists(C.GPU_LIB):
+        pytest.fail("tests/devapi/libdevapi_gpu.so not built")
+    import torch
+
+    assert torch.cuda.is_available()
+    torch.cuda.init()
+    return C.gpu()
+
+
+@pytest.fixture(scope="module")
+def checker():
+    ref = C.reference()
+    return ref if ref is not None else C.host()
+
+
+def _int_view(engine, slots):
+    return slots.astype(np.uint64 if engine in C.WORD64 else np.uint32)
+
+
+def _compare_slots(engine, name, got, want, mean=C.MEAN, sd=C.SD, ctx=""):
+    """got, want: [threads, calls, components] uint64 slots."""
+    if name in ("WORD", "WORDV", "SKIP"):
+        check_exact(_int_view(engine, got).ravel(), _int_view(engine, want).ravel())
+        return
+    if name.startswith(("POISSON", "DISCRETE")):
+        check_exact(got.astype(np.uint32).ravel(), want.astype(np.uint32).ravel())
+        return
+    if name == "MIXED":
+        calls = got.shape[1]
+        for r in range(5):
+            sel = [k for k in range(calls) if k % 5 == r]
+            if not sel:
+                continue
+            sub_name = {0: "NORMAL", 1: "UNIFORM", 2: "NORMAL_D", 3: "LOGN", 4: "NORMAL"}[r]
+            _compare_slots(engine, sub_name, got[:, sel, :1], want[:, sel, :1], mean, sd, ctx + f" k%5={r}")
+        return
+    if name in FLOAT_FNS:
+        g = got.astype(np.uint32).view(np.float32).ravel()
+        w = want.astype(np.uint32).view(np.float32).ravel()
+        suffix = "f32"
+    else:
+        g = got.view(np.float64).ravel()
+        w = want.astype(np.uint64).view(np.float64).ravel()
+        suffix = "f64"
+    if name.startswith("UNIFORM"):
+        check_exact(g, w)
+        return
+    log = name.startswith("LOGN")
+    kind = ("log_normal_" if log else "normal_") + suffix
+    params = (mean, sd) if log else (0.0, 1.0)
+    single = name in ("NORMAL", "NORMAL_D", "LOGN", "LOGN_D")
+    if single and engine in C.ONE_WORD:
+        check_erfinv(kind, g, w, *params)
+    else:
+        check_box_muller(kind, g, w, *params)
+
+
+@pytest.fixture(scope="module")
+def golden():
+    with np.load(GOLDEN, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("engine", C.ENGINES)
+def test_gpu_equals_reference_fixtures(gpu, golden, engine):
+    sub, off = golden["sub"], golden["off"]
+    for key, want in golden.items():
+        if not key.startswith(engine + "/"):
+            continue
+        parts = key.split("/")
+        name = parts[1]
+        lam = float(parts[2]) if len(parts) > 2 else None
+        print(f"case {key}", flush=True)
+        got = gpu.run(engine, C.F[name], sub, off, want.shape[1], a=lam if lam is not None else C.MEAN)
+        assert got is not None, key
+        _compare_slots(engine, name, got[:, :, :want.shape[2]], want.astype(np.uint64), ctx=key)
+
+
+@pytest.mark.parametrize("engine", C.ENGINES)
+def test_gpu_equals_checker_4096_threads(gpu, checker, engine):
+    n = 4096
+    sub, off = C.thread_params(n, seed=21)
+    table = C.discrete_table(size=257, seed=3)
+    for fn, name in enumerate(C.FNS):
+        lams = C.lambdas_for(engine) if name.startswith("POISSON") else [None]
+        for lam in lams:
+            calls = 2 if name.startswith("POISSON") and lam >= 64 else 8
+            a = lam if lam is not None else 0.25
+            print(f"case {engine} {name} lambda={lam}", flush=True)
+            want = checker.run(engine, fn, sub, off, calls, a=a, b=1.5, seed=0xC0FFEE, table=table)
+            got = gpu.run(engine, fn, sub, off, calls, a=a, b=1.5, seed=0xC0FFEE, table=table)
+            assert (got is None) == (want is None), name
+            if got is not None:
+                k = C.components(engine, fn)
+                _compare_slots(engine, name, got[:, :, :k], want[:, :, :k], mean=0.25, sd=1.5, ctx=name)
+
+
+@pytest.mark.parametrize("engine", ["lfsr113", "threefry2x64", "threefry4x32"])
+def test_gpu_inversion_poisson_between_500_and_1000(gpu, engine):
+    """No reference values here (its inversion does not terminate); equal to our host
+    branch and with Poisson moments."""
+    n, calls = 4096, 2
+    sub = np.arange(n, dtype=np.uint64)
+    off = np.full(n, 17, dtype=np.uint64)
+    for lam in (600.0, 999.0):
+        got = gpu.run(engine, C.F["POISSON"], sub, off, calls, a=lam)
+        want = C.host().run(engine, C.F["POISSON"], sub, off, calls, a=lam)
+        check_exact(got[:, :, 0].astype(np.uint32).ravel(), want[:, :, 0].astype(np.uint32).ravel())
+        k = got[:, :, 0].astype(np.float64).ravel()
+        assert abs(k.mean() - lam) < 5 * np.sqrt(lam / k.size)
+
+
+@pytest.mark.parametrize("engine,rng", [("philox", "PHILOX4_32_10"), ("threefry2x32", "THREEFRY2_32_20"),
+                                        ("threefry4x32", "THREEFRY4_32_20")])
+def test_device_states_continue_the_bulk_stream(gpu, engine, rng):
+    import torch
+
+    from rocrand_amd import rocrand as R
+
+    threads, calls = 64, 256
+    seed = 0x1234ABCD5678
+    g = R.PRNG(getattr(R, rng), seed=seed)
+    bulk = torch.empty(threads * calls, dtype=torch.uint32, device="cuda")
+    g.generate(bulk)
+    torch.cuda.synchronize()
+    off = np.arange(threads, dtype=np.uint64) * calls
+    got = gpu.run(engine, C.F["WORD"], np.zeros(threads, np.uint64), off, calls, seed=seed)[:, :, 0]
+    assert np.array_equal(got.astype(np.uint32).ravel(), bulk.cpu().numpy())
