@@ -3,8 +3,8 @@
  *
  * Same names, state types and call semantics as the reference's device API
  * (library/include/rocrand/rocrand_kernel.h and the engine / distribution headers it
- * includes), for Philox4x32-10, MRG32k3a, MRG31k3p, XORWOW, LFSR113 and ThreeFry
- * 2x32/2x64/4x32/4x64-20: rocrand_init, rocrand / rocrand2 / rocrand4, skipahead,
+ * includes), for Philox4x32-10, MRG32k3a, MRG31k3p, XORWOW, LFSR113, ThreeFry
+ * 2x32/2x64/4x32/4x64-20 and Sobol32/64 (plain and scrambled): rocrand_init, rocrand / rocrand2 / rocrand4, skipahead,
  * skipahead_subsequence, skipahead_sequence, rocrand_uniform*, rocrand_normal*,
  * rocrand_log_normal*, rocrand_poisson* and rocrand_discrete*.  A state initialised with
  * (seed, subsequence, offset) produces the words of the reference state initialised the
@@ -12,7 +12,9 @@
  * (rocrand_amd_engines.h, rocrand_amd_math.h), so a thread's normals equal the bulk
  * generator's for the same words.  Jump matrices: rocrand_amd_jumptables.h.
  *
- * Not provided here: the MTGP32 and Sobol device states.
+ * Sobol32/64 and their scrambled forms take the caller's direction vectors and scramble
+ * constants (rocrand_get_direction_vectors32/64, rocrand_get_scramble_constants32/64).
+ * Not provided here: the MTGP32 device state.
  */
 #ifndef ROCRAND_KERNEL_H_
 #define ROCRAND_KERNEL_H_
@@ -628,8 +630,121 @@ public:
     state_type m_state;
 };
 
+// ---------------------------------------------------------------- Sobol (quasi-random)
+// rocrand_sobol32.h / rocrand_sobol64.h: the state keeps a copy of the dimension's
+// direction vectors, the point index i and the current value d (the Gray-code XOR of the
+// vectors selected by i ^ (i >> 1)); each step XORs in the vector of i's lowest zero bit.
+template<class W, int BITS>
+struct sobol_state_words
+{
+    W d;
+    W i;
+    W vectors[BITS];
+};
+
+template<class W, int BITS>
+class sobol_engine
+{
+public:
+    using state_type = sobol_state_words<W, BITS>;
+
+    __host__ __device__ sobol_engine() : m_state() {}
+    __host__ __device__ sobol_engine(const W* vectors, W offset)
+    {
+        m_state.d = 0;
+        m_state.i = 0;
+        for(int k = 0; k < BITS; k++)
+            m_state.vectors[k] = vectors[k];
+        discard(offset);
+    }
+    // point i + offset: recomputed from its Gray code
+    __host__ __device__ void discard(W offset)
+    {
+        m_state.i += offset;
+        const W g = m_state.i ^ (m_state.i >> 1);
+        W       d = 0;
+        for(int k = 0; k < BITS; k++)
+            d ^= ((g >> k) & 1) ? m_state.vectors[k] : W(0);
+        m_state.d = d;
+    }
+    __host__ __device__ void discard()
+    {
+        m_state.d ^= m_state.vectors[lowest_zero_bit(m_state.i)];
+        m_state.i++;
+    }
+    // i += stride for a power of two stride, i a multiple of it (the lanes of a
+    // strided loop): the Gray codes of i and i + stride differ in bits log2(stride) - 1
+    // and the lowest zero bit of i | (stride - 1)
+    __host__ __device__ void discard_stride(W stride)
+    {
+        if(stride == 1)
+        {
+            discard();
+            return;
+        }
+        m_state.d ^= m_state.vectors[lowest_zero_bit(~stride) - 1];
+        m_state.d ^= m_state.vectors[lowest_zero_bit(m_state.i | (stride - 1))];
+        m_state.i += stride;
+    }
+    __host__ __device__ W operator()() { return next(); }
+    __host__ __device__ W next()
+    {
+        const W p = m_state.d;
+        discard();
+        return p;
+    }
+    __host__ __device__ W current() const { return m_state.d; }
+
+    state_type m_state;
+
+private:
+    // index of the lowest zero bit; 0 for an all-ones value (as the reference's device
+    // branch, whose ffs(~x) is 0 there)
+    __host__ __device__ static unsigned int lowest_zero_bit(W x)
+    {
+        const W y = ~x;
+        if(y == 0)
+            return 0;
+        if constexpr(BITS == 32)
+            return (unsigned int)__builtin_ctz((unsigned int)y);
+        else
+            return (unsigned int)__builtin_ctzll((unsigned long long)y);
+    }
+};
+typedef sobol_engine<unsigned int, 32>       sobol32_engine;
+typedef sobol_engine<unsigned long long, 64> sobol64_engine;
+
+// rocrand_scrambled_sobol32.h / 64.h: the Sobol value XOR a per-dimension constant.
+template<class W, int BITS>
+class scrambled_sobol_engine
+{
+public:
+    __host__ __device__ scrambled_sobol_engine() : m_engine(), scramble_constant() {}
+    __host__ __device__ scrambled_sobol_engine(const W* vectors, W scramble, W offset)
+        : m_engine(vectors, 0), scramble_constant(scramble)
+    {
+        m_engine.discard(offset);
+    }
+    __host__ __device__ void discard(W offset) { m_engine.discard(offset); }
+    __host__ __device__ void discard() { m_engine.discard(); }
+    __host__ __device__ void discard_stride(W stride) { m_engine.discard_stride(stride); }
+    __host__ __device__ W    operator()() { return next(); }
+    __host__ __device__ W    next() { return m_engine.next() ^ scramble_constant; }
+    __host__ __device__ W    current() const { return m_engine.current() ^ scramble_constant; }
+
+private:
+    sobol_engine<W, BITS> m_engine;
+    W                     scramble_constant;
+};
+typedef scrambled_sobol_engine<unsigned int, 32>       scrambled_sobol32_engine;
+typedef scrambled_sobol_engine<unsigned long long, 64> scrambled_sobol64_engine;
+
 } // namespace rocrand_device
 
+typedef rocrand_device::sobol32_engine           rocrand_state_sobol32;
+typedef rocrand_device::sobol64_engine           rocrand_state_sobol64;
+typedef rocrand_device::scrambled_sobol32_engine rocrand_state_scrambled_sobol32;
+typedef rocrand_device::scrambled_sobol64_engine rocrand_state_scrambled_sobol64;
 typedef rocrand_device::philox4x32_10_engine   rocrand_state_philox4x32_10;
 typedef rocrand_device::mrg32k3a_engine        rocrand_state_mrg32k3a;
 typedef rocrand_device::mrg31k3p_engine        rocrand_state_mrg31k3p;
@@ -734,6 +849,59 @@ ROCRAND_AMD_SKIP(rocrand_state_mrg31k3p, unsigned long long, discard_sequence)
 ROCRAND_AMD_SKIP(rocrand_state_xorwow, unsigned long long, discard_subsequence)
 ROCRAND_AMD_SKIP(rocrand_state_lfsr113, unsigned int, discard_subsequence)
 #undef ROCRAND_AMD_SKIP
+
+// Sobol: the caller passes the dimension's direction vectors (rocrand_get_direction_vectors32/64)
+// and, for the scrambled states, its constant (rocrand_get_scramble_constants32/64).
+__host__ __device__ __forceinline__ void rocrand_init(const unsigned int* vectors, const unsigned int offset,
+                                                      rocrand_state_sobol32* state)
+{
+    *state = rocrand_state_sobol32(vectors, offset);
+}
+__host__ __device__ __forceinline__ void rocrand_init(const unsigned long long* vectors, const unsigned int offset,
+                                                      rocrand_state_sobol64* state)
+{
+    *state = rocrand_state_sobol64(vectors, offset);
+}
+__host__ __device__ __forceinline__ void rocrand_init(const unsigned int* vectors, const unsigned int scramble_constant,
+                                                      const unsigned int               offset,
+                                                      rocrand_state_scrambled_sobol32* state)
+{
+    *state = rocrand_state_scrambled_sobol32(vectors, scramble_constant, offset);
+}
+__host__ __device__ __forceinline__ void rocrand_init(const unsigned long long*        vectors,
+                                                      const unsigned long long         scramble_constant,
+                                                      const unsigned int               offset,
+                                                      rocrand_state_scrambled_sobol64* state)
+{
+    *state = rocrand_state_scrambled_sobol64(vectors, scramble_constant, offset);
+}
+__host__ __device__ __forceinline__ unsigned int rocrand(rocrand_state_sobol32* state) { return state->next(); }
+__host__ __device__ __forceinline__ unsigned long long rocrand(rocrand_state_sobol64* state) { return state->next(); }
+__host__ __device__ __forceinline__ unsigned int rocrand(rocrand_state_scrambled_sobol32* state)
+{
+    return state->next();
+}
+__host__ __device__ __forceinline__ unsigned long long rocrand(rocrand_state_scrambled_sobol64* state)
+{
+    return state->next();
+}
+// the 32-bit states skip an unsigned int count (rocrand_sobol32.h skipahead -> discard)
+__host__ __device__ __forceinline__ void skipahead(unsigned long long offset, rocrand_state_sobol32* state)
+{
+    state->discard((unsigned int)offset);
+}
+__host__ __device__ __forceinline__ void skipahead(unsigned long long offset, rocrand_state_scrambled_sobol32* state)
+{
+    state->discard((unsigned int)offset);
+}
+__host__ __device__ __forceinline__ void skipahead(unsigned long long offset, rocrand_state_sobol64* state)
+{
+    state->discard(offset);
+}
+__host__ __device__ __forceinline__ void skipahead(unsigned long long offset, rocrand_state_scrambled_sobol64* state)
+{
+    state->discard(offset);
+}
 
 // ---------------------------------------------------------------- transforms
 // Device: the library's transforms (rocrand_amd_math.h); host: the reference's host
@@ -923,6 +1091,21 @@ __host__ __device__ __forceinline__ unsigned int discrete_alias(double x, const 
     i                      = i < d.size ? i : d.size - 1;
     return d.offset + (y < d.probability[i] ? i : d.alias[i]);
 }
+// rocrand_discrete.h:109-132: binary search of the CDF (the Sobol states use it so that
+// the sample stays monotone in the point)
+__host__ __device__ __forceinline__ unsigned int discrete_cdf(double x, const rocrand_discrete_distribution_st& d)
+{
+    unsigned int lo = 0, hi = d.size - 1;
+    while(lo != hi)
+    {
+        const unsigned int c = (lo + hi) / 2;
+        if(x > d.cdf[c])
+            lo = c + 1;
+        else
+            hi = c;
+    }
+    return d.offset + lo;
+}
 __host__ __device__ __forceinline__ double draw(unsigned int r) { return (double)r * R2P32D; }
 __host__ __device__ __forceinline__ double draw(unsigned long long r) { return (double)r * R2P64D; }
 
@@ -1069,6 +1252,10 @@ ROCRAND_AMD_ONE_WORD_UNIFORM(rocrand_state_threefry2x32_20)
 ROCRAND_AMD_ONE_WORD_UNIFORM(rocrand_state_threefry2x64_20)
 ROCRAND_AMD_ONE_WORD_UNIFORM(rocrand_state_threefry4x32_20)
 ROCRAND_AMD_ONE_WORD_UNIFORM(rocrand_state_threefry4x64_20)
+ROCRAND_AMD_ONE_WORD_UNIFORM(rocrand_state_sobol32)
+ROCRAND_AMD_ONE_WORD_UNIFORM(rocrand_state_sobol64)
+ROCRAND_AMD_ONE_WORD_UNIFORM(rocrand_state_scrambled_sobol32)
+ROCRAND_AMD_ONE_WORD_UNIFORM(rocrand_state_scrambled_sobol64)
 #undef ROCRAND_AMD_ONE_WORD_UNIFORM
 
 // ---------------------------------------------------------------- normal (rocrand_normal.h:340-1450)
@@ -1154,6 +1341,10 @@ ROCRAND_AMD_ONE_WORD_NORMAL(rocrand_state_threefry2x32_20)
 ROCRAND_AMD_ONE_WORD_NORMAL(rocrand_state_threefry2x64_20)
 ROCRAND_AMD_ONE_WORD_NORMAL(rocrand_state_threefry4x32_20)
 ROCRAND_AMD_ONE_WORD_NORMAL(rocrand_state_threefry4x64_20)
+ROCRAND_AMD_ONE_WORD_NORMAL(rocrand_state_sobol32)
+ROCRAND_AMD_ONE_WORD_NORMAL(rocrand_state_sobol64)
+ROCRAND_AMD_ONE_WORD_NORMAL(rocrand_state_scrambled_sobol32)
+ROCRAND_AMD_ONE_WORD_NORMAL(rocrand_state_scrambled_sobol64)
 #undef ROCRAND_AMD_ONE_WORD_NORMAL
 __host__ __device__ __forceinline__ float2 rocrand_normal2(rocrand_state_lfsr113* state)
 {
@@ -1318,6 +1509,21 @@ __host__ __device__ __forceinline__ double2 rocrand_log_normal_double2(rocrand_s
     const double2 r = rocrand_normal_double2(state);
     return double2{ln_d(mean, stddev, r.x), ln_d(mean, stddev, r.y)};
 }
+#define ROCRAND_AMD_LN_SOBOL(STATE)                                                                          \
+    __host__ __device__ __forceinline__ float rocrand_log_normal(STATE* state, float mean, float stddev)     \
+    {                                                                                                        \
+        return rocrand_device::amd_detail::ln_f(mean, stddev, rocrand_normal(state));                        \
+    }                                                                                                        \
+    __host__ __device__ __forceinline__ double rocrand_log_normal_double(STATE* state, double mean,          \
+                                                                         double stddev)                      \
+    {                                                                                                        \
+        return rocrand_device::amd_detail::ln_d(mean, stddev, rocrand_normal_double(state));                 \
+    }
+ROCRAND_AMD_LN_SOBOL(rocrand_state_sobol32)
+ROCRAND_AMD_LN_SOBOL(rocrand_state_sobol64)
+ROCRAND_AMD_LN_SOBOL(rocrand_state_scrambled_sobol32)
+ROCRAND_AMD_LN_SOBOL(rocrand_state_scrambled_sobol64)
+#undef ROCRAND_AMD_LN_SOBOL
 __host__ __device__ __forceinline__ float4 rocrand_log_normal4(rocrand_state_philox4x32_10* state, float mean,
                                                                float stddev)
 {
@@ -1396,12 +1602,12 @@ __host__ __device__ __forceinline__ unsigned int poisson_large(S* state, double 
             return (unsigned int)n;
     }
 }
-template<class S>
-__host__ __device__ __forceinline__ unsigned int poisson_huge(S* state, double lambda)
+template<class S, class R = unsigned int>
+__host__ __device__ __forceinline__ R poisson_huge(S* state, double lambda)
 {
     ROCRAND_AMD_FP_EXACT
     const double n = rocrand_normal_double(state);
-    return (unsigned int)round(sqrt(lambda) * n + lambda);
+    return (R)round(sqrt(lambda) * n + lambda);
 }
 template<class S>
 __host__ __device__ __forceinline__ unsigned int poisson(S* state, double lambda)
@@ -1413,20 +1619,20 @@ __host__ __device__ __forceinline__ unsigned int poisson(S* state, double lambda
     return poisson_huge(state, lambda);
 }
 // rocrand_poisson.h:164-216: inversion (Fishman's ITR) in blocks of e^-500 (LFSR113,
-// ThreeFry).  The scale of block j is e^-500 while lambda exceeds the block's end
+// ThreeFry, Sobol; R is the 64-bit count of the Sobol64 states).  The scale of block j is e^-500 while lambda exceeds the block's end
 // (500 (j + 1)) and e^(500 j - lambda) for the last block.  The reference compares lambda
 // with the first block's end only, so for 500 < lambda < 1000 its second block scales by
 // e^-500 again, its CDF tops out at e^(lambda - 1000) < 1 and a draw above that spins
 // until the 32-bit count wraps.  Here every block compares with its own end: the same
 // values for lambda <= 500 and lambda >= 1000, a terminating inversion in between.
-template<class S>
-__host__ __device__ __forceinline__ unsigned int poisson_inv(S* state, double lambda)
+template<class S, class R = unsigned int>
+__host__ __device__ __forceinline__ R poisson_inv(S* state, double lambda)
 {
     ROCRAND_AMD_FP_EXACT
     if(lambda >= 1000.0)
-        return poisson_huge(state, lambda);
+        return poisson_huge<S, R>(state, lambda);
     double       x = 1.0, y = 1.0, L;
-    unsigned int k  = 0;
+    R            k  = 0;
     int          pw = 0;
     const double u  = rocrand_uniform_double(state) - R2P32D / 2.0;
     const double ex = exp(-500.0);
@@ -1462,7 +1668,19 @@ ROCRAND_AMD_POISSON(rocrand_state_threefry2x32_20, poisson_inv)
 ROCRAND_AMD_POISSON(rocrand_state_threefry2x64_20, poisson_inv)
 ROCRAND_AMD_POISSON(rocrand_state_threefry4x32_20, poisson_inv)
 ROCRAND_AMD_POISSON(rocrand_state_threefry4x64_20, poisson_inv)
+ROCRAND_AMD_POISSON(rocrand_state_sobol32, poisson_inv)
+ROCRAND_AMD_POISSON(rocrand_state_scrambled_sobol32, poisson_inv)
 #undef ROCRAND_AMD_POISSON
+__host__ __device__ __forceinline__ unsigned long long rocrand_poisson(rocrand_state_sobol64* state, double lambda)
+{
+    return rocrand_device::amd_detail::poisson_inv<rocrand_state_sobol64, unsigned long long>(state, lambda);
+}
+__host__ __device__ __forceinline__ unsigned long long rocrand_poisson(rocrand_state_scrambled_sobol64* state,
+                                                                      double                           lambda)
+{
+    return rocrand_device::amd_detail::poisson_inv<rocrand_state_scrambled_sobol64, unsigned long long>(state,
+                                                                                                        lambda);
+}
 __host__ __device__ __forceinline__ uint4 rocrand_poisson4(rocrand_state_philox4x32_10* state, double lambda)
 {
     const unsigned int a = rocrand_poisson(state, lambda);
@@ -1490,6 +1708,18 @@ ROCRAND_AMD_DISCRETE(rocrand_state_threefry2x64_20)
 ROCRAND_AMD_DISCRETE(rocrand_state_threefry4x32_20)
 ROCRAND_AMD_DISCRETE(rocrand_state_threefry4x64_20)
 #undef ROCRAND_AMD_DISCRETE
+#define ROCRAND_AMD_DISCRETE_CDF(STATE)                                                                      \
+    __host__ __device__ __forceinline__ unsigned int rocrand_discrete(                                       \
+        STATE* state, const rocrand_discrete_distribution discrete_distribution)                             \
+    {                                                                                                        \
+        using namespace rocrand_device::amd_detail;                                                          \
+        return discrete_cdf(draw(rocrand(state)), *discrete_distribution);                                   \
+    }
+ROCRAND_AMD_DISCRETE_CDF(rocrand_state_sobol32)
+ROCRAND_AMD_DISCRETE_CDF(rocrand_state_sobol64)
+ROCRAND_AMD_DISCRETE_CDF(rocrand_state_scrambled_sobol32)
+ROCRAND_AMD_DISCRETE_CDF(rocrand_state_scrambled_sobol64)
+#undef ROCRAND_AMD_DISCRETE_CDF
 __host__ __device__ __forceinline__ uint4 rocrand_discrete4(rocrand_state_philox4x32_10*        state,
                                                             const rocrand_discrete_distribution discrete_distribution)
 {

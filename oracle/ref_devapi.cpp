@@ -19,7 +19,8 @@
 extern "C" int ref_devapi_run(int engine, int fn, unsigned long long seed, const unsigned long long* sub,
                               const unsigned long long* off, int n, int calls, double a, double b,
                               unsigned int dsize, unsigned int doffset, const double* prob,
-                              const unsigned int* alias, unsigned long long* out)
+                              const unsigned int* alias, const double* cdf, const unsigned int* v32,
+                              const unsigned long long* v64, unsigned int dims, unsigned long long* out)
 {
     if(n < 0 || calls < 0 || ((fn == devapi_test::DISCRETE || fn == devapi_test::DISCRETE4) && dsize == 0))
         return -2;
@@ -28,6 +29,9 @@ extern "C" int ref_devapi_run(int engine, int fn, unsigned long long seed, const
     d.offset      = doffset;
     d.alias       = const_cast<unsigned int*>(alias);
     d.probability = const_cast<double*>(prob);
-    d.cdf         = nullptr;
-    return devapi_test::run_host(engine, fn, seed, sub, off, n, calls, a, b, &d, (uint64_t*)out);
+    d.cdf         = const_cast<double*>(cdf);
+    const devapi_test::sobol_tables tables = {v32, v64, dims == 0 ? 1u : dims};
+    if(engine >= devapi_test::SOBOL32 && (dims == 0 || !v32 || !v64))
+        return -2;
+    return devapi_test::run_host(engine, fn, seed, sub, off, n, calls, a, b, &d, tables, (uint64_t*)out);
 }

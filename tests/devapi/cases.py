@@ -14,15 +14,17 @@ HOST_LIB = os.path.join(HERE, "libdevapi_host.so")
 GPU_LIB = os.path.join(HERE, "libdevapi_gpu.so")
 
 ENGINES = ["philox", "mrg32k3a", "mrg31k3p", "xorwow", "lfsr113", "threefry2x32", "threefry2x64",
-           "threefry4x32", "threefry4x64"]
+           "threefry4x32", "threefry4x64", "sobol32", "sobol64", "scrambled_sobol32", "scrambled_sobol64"]
 FNS = ["WORD", "WORDV", "UNIFORM", "UNIFORM2", "UNIFORM4", "UNIFORM_D", "UNIFORM_D2", "UNIFORM_D4", "NORMAL",
        "NORMAL2", "NORMAL4", "NORMAL_D", "NORMAL_D2", "NORMAL_D4", "LOGN", "LOGN2", "LOGN4", "LOGN_D", "LOGN_D2",
        "LOGN_D4", "POISSON", "POISSON4", "DISCRETE", "DISCRETE4", "MIXED", "SKIP"]
 E = {n: i for i, n in enumerate(ENGINES)}
 F = {n: i for i, n in enumerate(FNS)}
 # engines whose single normals use the inverse error function and whose Poisson inverts
-ONE_WORD = {"lfsr113", "threefry2x32", "threefry2x64", "threefry4x32", "threefry4x64"}
-WORD64 = {"threefry2x64", "threefry4x64"}
+ONE_WORD = {"lfsr113", "threefry2x32", "threefry2x64", "threefry4x32", "threefry4x64", "sobol32", "sobol64",
+            "scrambled_sobol32", "scrambled_sobol64"}
+WORD64 = {"threefry2x64", "threefry4x64", "sobol64", "scrambled_sobol64"}
+SOBOL = {"sobol32", "sobol64", "scrambled_sobol32", "scrambled_sobol64"}
 
 SEED = 0x0123456789ABCDEF
 MEAN, SD = 0.3, 0.7
@@ -51,9 +53,21 @@ def thread_params(n, seed=7):
 
 
 def discrete_table(size=37, seed=11):
-    """Any probability in [0, 1) and alias below size: parity concerns the sampler only."""
+    """(probability, alias, cdf, offset): any probability in [0, 1) and alias below size
+    (parity concerns the sampler only) and an increasing CDF ending at 1."""
     rng = np.random.default_rng(seed)
-    return rng.random(size), rng.integers(0, size, size).astype(np.uint32), 5
+    w = rng.random(size) + 0.01
+    cdf = np.cumsum(w) / w.sum()
+    cdf[-1] = 1.0
+    return rng.random(size), rng.integers(0, size, size).astype(np.uint32), cdf, 5
+
+
+def sobol_vectors(dims=16, seed=13):
+    """Random direction vectors: the state arithmetic is the same for any table."""
+    rng = np.random.default_rng(seed)
+    v32 = rng.integers(0, 2**32, dims * 32, dtype=np.uint64).astype(np.uint32)
+    v64 = rng.integers(0, 2**64 - 1, dims * 64, dtype=np.uint64, endpoint=True)
+    return v32, v64
 
 
 def components(engine, fn):
@@ -72,7 +86,8 @@ def _bind(lib, name):
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                   ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint, ctypes.c_uint, ctypes.c_void_p,
-                  ctypes.c_void_p, ctypes.c_void_p]
+                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint,
+                  ctypes.c_void_p]
     return f
 
 
@@ -84,16 +99,27 @@ class Driver:
         self.lib = ctypes.CDLL(path)
         self.fn = _bind(self.lib, symbol)
 
-    def run(self, engine, fn, sub, off, calls, a=MEAN, b=SD, seed=SEED, table=None):
+    def run(self, engine, fn, sub, off, calls, a=MEAN, b=SD, seed=SEED, table=None, vectors=None):
         n = len(sub)
         sub = np.ascontiguousarray(sub, dtype=np.uint64)
         off = np.ascontiguousarray(off, dtype=np.uint64)
-        prob, alias, doff = table if table is not None else discrete_table()
+        if engine in ("sobol32", "scrambled_sobol32"):
+            # keep the 32-bit point index below 2^32 - 256: at index 2^32 - 1 the
+            # reference's host branch reads past the direction vectors
+            # (rocrand_sobol32.h rightmost_zero_bit returns 32 for an all-ones index)
+            off = off.copy()
+            off[(off & np.uint64(0xFFFFFFFF)) >= np.uint64(0xFFFFFF00)] ^= np.uint64(0x100)
+        prob, alias, cdf, doff = table if table is not None else discrete_table()
         prob = np.ascontiguousarray(prob, dtype=np.float64)
         alias = np.ascontiguousarray(alias, dtype=np.uint32)
+        cdf = np.ascontiguousarray(cdf, dtype=np.float64)
+        v32, v64 = vectors if vectors is not None else sobol_vectors()
+        v32 = np.ascontiguousarray(v32, dtype=np.uint32)
+        v64 = np.ascontiguousarray(v64, dtype=np.uint64)
         out = np.zeros(n * calls * 4, dtype=np.uint64)
         rc = self.fn(E[engine], fn, seed, sub.ctypes.data, off.ctypes.data, n, calls, a, b, len(prob), doff,
-                     prob.ctypes.data, alias.ctypes.data, out.ctypes.data)
+                     prob.ctypes.data, alias.ctypes.data, cdf.ctypes.data, v32.ctypes.data, v64.ctypes.data,
+                     len(v32) // 32, out.ctypes.data)
         if rc == -1:
             return None
         assert rc == 0, f"driver returned {rc}"

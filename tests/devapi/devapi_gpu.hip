@@ -14,17 +14,18 @@ constexpr int BLOCK = 128;
 template<class S, int FN>
 __global__ __launch_bounds__(BLOCK) void devapi_kernel(unsigned long long seed, const unsigned long long* sub,
                                                        const unsigned long long* off, int n, int calls, double a,
-                                                       double b, rocrand_discrete_distribution dist, uint64_t* out)
+                                                       double b, rocrand_discrete_distribution dist,
+                                                       devapi_test::sobol_tables tables, uint64_t* out)
 {
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     if(t >= n)
         return;
-    devapi_test::run_thread<S, FN>(seed, sub[t], off[t], calls, a, b, dist, out + (size_t)t * calls * 4);
+    devapi_test::run_thread<S, FN>(seed, sub[t], off[t], calls, a, b, dist, tables, out + (size_t)t * calls * 4);
 }
 
 struct device_buffers
 {
-    void* p[6] = {};
+    void* p[9] = {};
     ~device_buffers()
     {
         for(void* q : p)
@@ -38,10 +39,13 @@ struct device_buffers
 extern "C" int devapi_gpu_run(int engine, int fn, unsigned long long seed, const unsigned long long* sub,
                               const unsigned long long* off, int n, int calls, double a, double b,
                               unsigned int dsize, unsigned int doffset, const double* prob,
-                              const unsigned int* alias, unsigned long long* out)
+                              const unsigned int* alias, const double* cdf, const unsigned int* v32,
+                              const unsigned long long* v64, unsigned int dims, unsigned long long* out)
 {
     const bool discrete = fn == devapi_test::DISCRETE || fn == devapi_test::DISCRETE4;
-    if(n <= 0 || calls <= 0 || (size_t)n * calls > (1u << 26) || (discrete && (dsize == 0 || !prob || !alias)))
+    const bool sobol    = engine >= devapi_test::SOBOL32;
+    if(n <= 0 || calls <= 0 || (size_t)n * calls > (1u << 26)
+       || (discrete && (dsize == 0 || !prob || !alias || !cdf)) || (sobol && (dims == 0 || !v32 || !v64)))
         return -2;
     bool known = false;
     devapi_test::with_engine(engine,
@@ -56,18 +60,22 @@ extern "C" int devapi_gpu_run(int engine, int fn, unsigned long long seed, const
     device_buffers     buf;
     const size_t       out_bytes = (size_t)n * calls * 4 * sizeof(uint64_t);
     const unsigned int dn        = discrete ? dsize : 1;
+    const unsigned int nd        = sobol ? dims : 1;
     if(hipMalloc(&buf.p[0], n * sizeof(unsigned long long)) != hipSuccess
        || hipMalloc(&buf.p[1], n * sizeof(unsigned long long)) != hipSuccess
        || hipMalloc(&buf.p[2], out_bytes) != hipSuccess || hipMalloc(&buf.p[3], dn * sizeof(double)) != hipSuccess
        || hipMalloc(&buf.p[4], dn * sizeof(unsigned int)) != hipSuccess
-       || hipMalloc(&buf.p[5], sizeof(rocrand_discrete_distribution_st)) != hipSuccess)
+       || hipMalloc(&buf.p[5], sizeof(rocrand_discrete_distribution_st)) != hipSuccess
+       || hipMalloc(&buf.p[6], dn * sizeof(double)) != hipSuccess
+       || hipMalloc(&buf.p[7], (size_t)nd * 32 * sizeof(unsigned int)) != hipSuccess
+       || hipMalloc(&buf.p[8], (size_t)nd * 64 * sizeof(unsigned long long)) != hipSuccess)
         return -3;
     rocrand_discrete_distribution_st d;
     d.size        = dn;
     d.offset      = doffset;
     d.alias       = (unsigned int*)buf.p[4];
     d.probability = (double*)buf.p[3];
-    d.cdf         = nullptr;
+    d.cdf         = (double*)buf.p[6];
     if(hipMemcpy(buf.p[0], sub, n * sizeof(unsigned long long), hipMemcpyHostToDevice) != hipSuccess
        || hipMemcpy(buf.p[1], off, n * sizeof(unsigned long long), hipMemcpyHostToDevice) != hipSuccess
        || hipMemset(buf.p[2], 0, out_bytes) != hipSuccess
@@ -75,8 +83,15 @@ extern "C" int devapi_gpu_run(int engine, int fn, unsigned long long seed, const
         return -3;
     if(discrete
        && (hipMemcpy(buf.p[3], prob, dn * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
-           || hipMemcpy(buf.p[4], alias, dn * sizeof(unsigned int), hipMemcpyHostToDevice) != hipSuccess))
+           || hipMemcpy(buf.p[4], alias, dn * sizeof(unsigned int), hipMemcpyHostToDevice) != hipSuccess
+           || hipMemcpy(buf.p[6], cdf, dn * sizeof(double), hipMemcpyHostToDevice) != hipSuccess))
         return -3;
+    if(sobol
+       && (hipMemcpy(buf.p[7], v32, (size_t)nd * 32 * sizeof(unsigned int), hipMemcpyHostToDevice) != hipSuccess
+           || hipMemcpy(buf.p[8], v64, (size_t)nd * 64 * sizeof(unsigned long long), hipMemcpyHostToDevice)
+                  != hipSuccess))
+        return -3;
+    const devapi_test::sobol_tables tables = {(const unsigned int*)buf.p[7], (const unsigned long long*)buf.p[8], nd};
 
     const unsigned int blocks = (unsigned int)((n + BLOCK - 1) / BLOCK);
     devapi_test::with_engine(
@@ -92,7 +107,7 @@ extern "C" int devapi_gpu_run(int engine, int fn, unsigned long long seed, const
                                          hipLaunchKernelGGL((devapi_kernel<S, FN>), dim3(blocks), dim3(BLOCK), 0, 0,
                                                             seed, (const unsigned long long*)buf.p[0],
                                                             (const unsigned long long*)buf.p[1], n, calls, a, b,
-                                                            (rocrand_discrete_distribution)buf.p[5],
+                                                            (rocrand_discrete_distribution)buf.p[5], tables,
                                                             (uint64_t*)buf.p[2]);
                                  });
         });

@@ -6,7 +6,8 @@
  *   - host code over include/rocrand/rocrand_kernel.h    (tests/devapi/devapi_host.cpp)
  *   - host code over the reference's rocrand_kernel.h    (oracle/ref_devapi.cpp)
  *
- * Thread t initialises its state with (seed, sub[t], off[t]) and makes `calls` calls of
+ * Thread t initialises its state with (seed, sub[t], off[t]) (a Sobol state: dimension
+ * sub[t] mod dims of the given direction vectors, offset off[t]) and makes `calls` calls of
  * one function; call k writes its result components into out[k * 4 + c] as 64-bit slots
  * (32-bit integers and floats as their bit patterns, zero-extended).  Functions a state
  * type does not have are reported by supported() and never instantiated.
@@ -32,6 +33,10 @@ enum engine_id
     TF2X64,
     TF4X32,
     TF4X64,
+    SOBOL32,
+    SOBOL64,
+    SSOBOL32, // scrambled
+    SSOBOL64,
     ENGINE_COUNT
 };
 
@@ -113,6 +118,34 @@ struct state_of<TF4X64>
 {
     using type = rocrand_state_threefry4x64_20;
 };
+template<>
+struct state_of<SOBOL32>
+{
+    using type = rocrand_state_sobol32;
+};
+template<>
+struct state_of<SOBOL64>
+{
+    using type = rocrand_state_sobol64;
+};
+template<>
+struct state_of<SSOBOL32>
+{
+    using type = rocrand_state_scrambled_sobol32;
+};
+template<>
+struct state_of<SSOBOL64>
+{
+    using type = rocrand_state_scrambled_sobol64;
+};
+
+// Direction vectors of the Sobol states: `dims` dimensions of 32 (or 64) words.
+struct sobol_tables
+{
+    const unsigned int*       v32;
+    const unsigned long long* v64;
+    unsigned int              dims;
+};
 
 // ---- which calls a state type has
 #define DEVAPI_HAS(NAME, EXPR)                                                                               \
@@ -128,6 +161,10 @@ DEVAPI_HAS(has_uniform2, rocrand_uniform2(std::declval<S*>()))
 DEVAPI_HAS(has_uniform4, rocrand_uniform4(std::declval<S*>()))
 DEVAPI_HAS(has_uniform_d2, rocrand_uniform_double2(std::declval<S*>()))
 DEVAPI_HAS(has_uniform_d4, rocrand_uniform_double4(std::declval<S*>()))
+DEVAPI_HAS(has_normal2, rocrand_normal2(std::declval<S*>()))
+DEVAPI_HAS(has_normal_d2, rocrand_normal_double2(std::declval<S*>()))
+DEVAPI_HAS(has_logn2, rocrand_log_normal2(std::declval<S*>(), 0.0f, 1.0f))
+DEVAPI_HAS(has_logn_d2, rocrand_log_normal_double2(std::declval<S*>(), 0.0, 1.0))
 DEVAPI_HAS(has_normal4, rocrand_normal4(std::declval<S*>()))
 DEVAPI_HAS(has_normal_d4, rocrand_normal_double4(std::declval<S*>()))
 DEVAPI_HAS(has_logn4, rocrand_log_normal4(std::declval<S*>(), 0.0f, 1.0f))
@@ -135,6 +172,7 @@ DEVAPI_HAS(has_logn_d4, rocrand_log_normal_double4(std::declval<S*>(), 0.0, 1.0)
 DEVAPI_HAS(has_poisson4, rocrand_poisson4(std::declval<S*>(), 1.0))
 DEVAPI_HAS(has_discrete4, rocrand_discrete4(std::declval<S*>(), std::declval<rocrand_discrete_distribution>()))
 DEVAPI_HAS(has_skipahead, skipahead(1ull, std::declval<S*>()))
+DEVAPI_HAS(has_skip_subsequence, skipahead_subsequence(1u, std::declval<S*>()))
 #undef DEVAPI_HAS
 
 template<class S>
@@ -147,6 +185,10 @@ constexpr bool supported(int fn)
         case UNIFORM4: return has_uniform4<S>::value;
         case UNIFORM_D2: return has_uniform_d2<S>::value;
         case UNIFORM_D4: return has_uniform_d4<S>::value;
+        case NORMAL2: return has_normal2<S>::value;
+        case NORMAL_D2: return has_normal_d2<S>::value;
+        case LOGN2: return has_logn2<S>::value;
+        case LOGN_D2: return has_logn_d2<S>::value;
         case NORMAL4: return has_normal4<S>::value;
         case NORMAL_D4: return has_normal_d4<S>::value;
         case LOGN4: return has_logn4<S>::value;
@@ -187,8 +229,30 @@ __host__ __device__ inline void put4(uint64_t* o, const V& v)
     o[3] = slot(v.w);
 }
 
+// Sobol: dimension sub mod dims; the scramble constant derives from the seed and the dimension
+__host__ __device__ inline void init(rocrand_state_sobol32* s, unsigned long long, unsigned long long sub,
+                                     unsigned long long off, const sobol_tables& t)
+{
+    rocrand_init(t.v32 + (sub % t.dims) * 32, (unsigned int)off, s);
+}
+__host__ __device__ inline void init(rocrand_state_sobol64* s, unsigned long long, unsigned long long sub,
+                                     unsigned long long off, const sobol_tables& t)
+{
+    rocrand_init(t.v64 + (sub % t.dims) * 64, (unsigned int)off, s);
+}
+__host__ __device__ inline void init(rocrand_state_scrambled_sobol32* s, unsigned long long seed,
+                                     unsigned long long sub, unsigned long long off, const sobol_tables& t)
+{
+    rocrand_init(t.v32 + (sub % t.dims) * 32, (unsigned int)(seed ^ (sub * 0x9E3779B97F4A7C15ull)), (unsigned int)off,
+                 s);
+}
+__host__ __device__ inline void init(rocrand_state_scrambled_sobol64* s, unsigned long long seed,
+                                     unsigned long long sub, unsigned long long off, const sobol_tables& t)
+{
+    rocrand_init(t.v64 + (sub % t.dims) * 64, seed ^ (sub * 0x9E3779B97F4A7C15ull), (unsigned int)off, s);
+}
 __host__ __device__ inline void init(rocrand_state_lfsr113* s, unsigned long long seed, unsigned long long sub,
-                                     unsigned long long off)
+                                     unsigned long long off, const sobol_tables&)
 {
     // the seed words must exceed 1, 7, 15 and 127 for the full period
     const uint4 seed4 = {(unsigned int)seed | 2u, (unsigned int)(seed >> 32) | 8u,
@@ -196,7 +260,8 @@ __host__ __device__ inline void init(rocrand_state_lfsr113* s, unsigned long lon
     rocrand_init(seed4, (unsigned int)sub, off, s);
 }
 template<class S>
-__host__ __device__ inline void init(S* s, unsigned long long seed, unsigned long long sub, unsigned long long off)
+__host__ __device__ inline void init(S* s, unsigned long long seed, unsigned long long sub, unsigned long long off,
+                                     const sobol_tables&)
 {
     rocrand_init(seed, sub, off, s);
 }
@@ -271,12 +336,17 @@ __host__ __device__ inline void call(S* s, int k, double a, double b, rocrand_di
     }
     else if constexpr(FN == SKIP)
     {
-        switch(k % 3)
+        if constexpr(has_skip_subsequence<S>::value)
         {
-            case 0: skipahead(((unsigned long long)(k + 1) << 37) + 1000003ull * (unsigned long long)k, s); break;
-            case 1: skipahead_subsequence((unsigned int)(k + 1), s); break;
-            default: skipahead_sequence((unsigned int)(k + 1), s); break;
+            switch(k % 3)
+            {
+                case 0: skipahead(((unsigned long long)(k + 1) << 37) + 1000003ull * (unsigned long long)k, s); break;
+                case 1: skipahead_subsequence((unsigned int)(k + 1), s); break;
+                default: skipahead_sequence((unsigned int)(k + 1), s); break;
+            }
         }
+        else
+            skipahead(((unsigned long long)(k + 1) << 37) + 1000003ull * (unsigned long long)k, s);
         o[0] = slot(rocrand(s));
     }
 }
@@ -285,10 +355,10 @@ __host__ __device__ inline void call(S* s, int k, double a, double b, rocrand_di
 template<class S, int FN>
 __host__ __device__ inline void run_thread(unsigned long long seed, unsigned long long sub, unsigned long long off,
                                            int calls, double a, double b, rocrand_discrete_distribution dist,
-                                           uint64_t* out)
+                                           const sobol_tables& tables, uint64_t* out)
 {
     S s;
-    init(&s, seed, sub, off);
+    init(&s, seed, sub, off, tables);
     for(int k = 0; k < calls; k++)
         call<S, FN>(&s, k, a, b, dist, out + (size_t)k * 4);
 }
@@ -347,6 +417,10 @@ inline bool with_engine(int engine, F&& f)
         case TF2X64: f((state_of<TF2X64>::type*)nullptr); return true;
         case TF4X32: f((state_of<TF4X32>::type*)nullptr); return true;
         case TF4X64: f((state_of<TF4X64>::type*)nullptr); return true;
+        case SOBOL32: f((state_of<SOBOL32>::type*)nullptr); return true;
+        case SOBOL64: f((state_of<SOBOL64>::type*)nullptr); return true;
+        case SSOBOL32: f((state_of<SSOBOL32>::type*)nullptr); return true;
+        case SSOBOL64: f((state_of<SSOBOL64>::type*)nullptr); return true;
         default: return false;
     }
 }
@@ -354,7 +428,7 @@ inline bool with_engine(int engine, F&& f)
 // The host run of every thread: returns -1 for an unknown or unsupported pair.
 inline int run_host(int engine, int fn, unsigned long long seed, const unsigned long long* sub,
                     const unsigned long long* off, int n, int calls, double a, double b,
-                    rocrand_discrete_distribution dist, uint64_t* out)
+                    rocrand_discrete_distribution dist, const sobol_tables& tables, uint64_t* out)
 {
     int rc = -1;
     with_engine(engine,
@@ -370,7 +444,7 @@ inline int run_host(int engine, int fn, unsigned long long seed, const unsigned 
                                 if constexpr(supported<S>(FN))
                                 {
                                     for(int t = 0; t < n; t++)
-                                        run_thread<S, FN>(seed, sub[t], off[t], calls, a, b, dist,
+                                        run_thread<S, FN>(seed, sub[t], off[t], calls, a, b, dist, tables,
                                                           out + (size_t)t * calls * 4);
                                     rc = 0;
                                 }

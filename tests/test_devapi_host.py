@@ -9,7 +9,9 @@ branch (tests/devapi/libdevapi_host.so) must reproduce, bit for bit:
 * the same reference build live (oracle/_ref/libref.so, where the reference sources are
   present) on random seeds, subsequences and offsets, for every engine and call;
 * the oracle's engine streams (oracle/orc_engines.c): rocrand() on a state initialised
-  with (seed, subsequence, offset) is the engine's word stream from that position.
+  with (seed, subsequence, offset) is the engine's word stream from that position;
+* for Sobol, the Gray-code definition: point i of dimension j is the XOR of the direction
+  vectors selected by the bits of i ^ (i >> 1) (XOR the scramble constant).
 
 For 500 < lambda < 1000 the reference's inversion Poisson (LFSR113, ThreeFry) does not
 terminate in useful time; ours completes the inversion, checked here against the Poisson
@@ -54,9 +56,13 @@ def _golden_cases(golden):
 def test_golden_covers_every_engine_and_call(golden):
     keys = {tuple(k.split("/")[:2]) for k in golden if "/" in k}
     for engine in C.ENGINES:
-        for name in ("WORD", "UNIFORM", "UNIFORM_D", "NORMAL", "NORMAL2", "NORMAL_D", "NORMAL_D2", "LOGN", "LOGN2",
-                     "LOGN_D", "LOGN_D2", "POISSON", "DISCRETE", "MIXED"):
+        for name in ("WORD", "UNIFORM", "UNIFORM_D", "NORMAL", "NORMAL_D", "LOGN", "LOGN_D", "POISSON", "DISCRETE",
+                     "MIXED"):
             assert (engine, name) in keys, (engine, name)
+        pairs = [(engine, n) in keys for n in ("NORMAL2", "NORMAL_D2", "LOGN2", "LOGN_D2")]
+        assert all(pairs) if engine not in C.SOBOL else not any(pairs), engine
+    for engine in C.SOBOL:
+        assert (engine, "SKIP") in keys and (engine, "WORDV") not in keys
     # Philox's vector calls and the skipahead family where the reference has them
     for name in ("WORDV", "UNIFORM2", "UNIFORM4", "UNIFORM_D2", "UNIFORM_D4", "NORMAL4", "NORMAL_D4", "LOGN4",
                  "LOGN_D4", "POISSON4", "DISCRETE4", "SKIP"):
@@ -114,7 +120,8 @@ def test_words_are_the_oracle_engine_stream(host, oracle, engine, rng_type):
         assert np.array_equal(got[t].astype(np.uint32), want), f"{engine} thread {t}"
 
 
-@pytest.mark.parametrize("engine", ["mrg32k3a", "mrg31k3p", "xorwow", "lfsr113", "philox"])
+@pytest.mark.parametrize("engine", ["mrg32k3a", "mrg31k3p", "xorwow", "lfsr113", "philox", "sobol32", "sobol64",
+                                    "scrambled_sobol64"])
 def test_skipahead_equals_stepping(host, engine):
     """skipahead(k) then rocrand() is word k of the unskipped stream (offset 0 + k)."""
     calls = 1
@@ -136,6 +143,32 @@ def test_inversion_poisson_between_500_and_1000(host, engine, lam):
     assert abs(k.mean() - lam) < 5 * np.sqrt(lam / m)
     assert abs(k.var() / lam - 1.0) < 5 * np.sqrt(2.0 / m)
     assert k.min() > lam - 8 * np.sqrt(lam)
+
+
+@pytest.mark.parametrize("engine,bits", [("sobol32", 32), ("sobol64", 64), ("scrambled_sobol32", 32),
+                                         ("scrambled_sobol64", 64)])
+def test_sobol_points_are_gray_code_xors(host, engine, bits):
+    v32, v64 = C.sobol_vectors(dims=4, seed=2)
+    vec = (v32 if bits == 32 else v64).reshape(4, bits).astype(object)
+    sub = np.array([0, 1, 2, 3, 5], dtype=np.uint64)
+    off = np.array([0, 1, 77, 2**20 + 3, 2**31 - 2], dtype=np.uint64)
+    calls = 40
+    got = host.run(engine, C.F["WORD"], sub, off, calls, seed=0xABCDEF0123, vectors=(v32, v64))[:, :, 0]
+    mask = (1 << bits) - 1
+    for t in range(len(sub)):
+        dim = int(sub[t]) % 4
+        scr = 0
+        if engine.startswith("scrambled"):
+            scr = (0xABCDEF0123 ^ (int(sub[t]) * 0x9E3779B97F4A7C15)) & mask
+            scr &= 0xFFFFFFFF if bits == 32 else mask
+        for k in range(calls):
+            i = int(off[t]) + k
+            g = i ^ (i >> 1)
+            d = 0
+            for b in range(bits):
+                if (g >> b) & 1:
+                    d ^= int(vec[dim, b])
+            assert int(got[t, k]) == d ^ scr, (engine, t, k)
 
 
 def test_gpu_driver_exports():

@@ -161,3 +161,39 @@ def test_device_states_continue_the_bulk_stream(gpu, engine, rng):
     off = np.arange(threads, dtype=np.uint64) * calls
     got = gpu.run(engine, C.F["WORD"], np.zeros(threads, np.uint64), off, calls, seed=seed)[:, :, 0]
     assert np.array_equal(got.astype(np.uint32).ravel(), bulk.cpu().numpy())
+
+
+@pytest.mark.parametrize("engine,rng,bits", [("sobol32", "SOBOL32", 32), ("sobol64", "SOBOL64", 64)])
+def test_sobol_states_equal_the_bulk_quasi_generator(gpu, engine, rng, bits):
+    """States built from rocrand_get_direction_vectors32/64 (the Joe-Kuo tables the bulk
+    generator uses) at offset O give point O + i of their dimension, which is what
+    rocrand_generate writes at [dimension * n + i] for a generator with that offset."""
+    import ctypes
+
+    import torch
+
+    from rocrand_amd import rocrand as R
+
+    L = R.load()
+    dims, n, offset = 6, 1000, 12345
+    if bits == 32:
+        p = ctypes.POINTER(ctypes.c_uint)()
+        R.check_rocrand(L.rocrand_get_direction_vectors32(ctypes.byref(p), 101))
+        v = np.ctypeslib.as_array(p, shape=(dims * 32,)).copy()
+        vectors = (v, np.zeros(dims * 64, np.uint64))
+        dtype = torch.uint32
+    else:
+        p = ctypes.POINTER(ctypes.c_ulonglong)()
+        R.check_rocrand(L.rocrand_get_direction_vectors64(ctypes.byref(p), 103))
+        v = np.ctypeslib.as_array(p, shape=(dims * 64,)).copy()
+        vectors = (np.zeros(dims * 32, np.uint32), v)
+        dtype = torch.uint64
+    g = R.QRNG(getattr(R, rng), ndim=dims, offset=offset)
+    bulk = torch.empty(dims * n, dtype=dtype, device="cuda")
+    g.generate(bulk)
+    torch.cuda.synchronize()
+    sub = np.arange(dims, dtype=np.uint64)
+    off = np.full(dims, offset, dtype=np.uint64)
+    got = gpu.run(engine, C.F["WORD"], sub, off, n, vectors=vectors)[:, :, 0]
+    want = bulk.cpu().numpy().reshape(dims, n).astype(np.uint64)
+    assert np.array_equal(got, want)
