@@ -2,6 +2,7 @@
 """Benchmark of the bulk-generation hot path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-table] [--no-cpu]
+    python bench.py --only ENGINE:KIND --steps K --warmup W      (one pair, for profiler runs)
 
 A step is one rocrand_generate_uniform call (Philox4x32-10, seed 12345,
 offset 0) writing N = 2^28 fp32 samples into a preallocated HBM buffer: the
@@ -47,8 +48,7 @@ def time_calls(fn, stream, steps, warmup):
     return start.elapsed_time(end) / steps
 
 
-def per_generator_table(R, torch, stream, n, steps):
-    """GB/s for each engine x distribution at n samples (same workload as the step)."""
+def _pairs(R, torch):
     engines = [("philox4x32_10", R.PHILOX4_32_10), ("threefry4x32_20", R.THREEFRY4_32_20),
                ("mrg32k3a", R.MRG32K3A), ("xorwow", R.XORWOW), ("lfsr113", R.LFSR113),
                ("mrg31k3p", R.MRG31K3P), ("threefry2x32_20", R.THREEFRY2_32_20),
@@ -62,6 +62,26 @@ def per_generator_table(R, torch, stream, n, steps):
              ("poisson_10", torch.uint32, lambda g, b: g.poisson(b, 10.0))]
     engines += [("mtgp32", R.MTGP32), ("mt19937", R.MT19937), ("sobol32", R.SOBOL32), ("scrambled_sobol32", R.SCRAMBLED_SOBOL32),
                 ("sobol64", R.SOBOL64), ("scrambled_sobol64", R.SCRAMBLED_SOBOL64)]
+    return engines, dists
+
+
+def only_pair(R, torch, stream, n, pair, steps, warmup):
+    """GB/s of one ENGINE:KIND pair of the per-generator table (for profiler runs)."""
+    engines, dists = _pairs(R, torch)
+    ename, dname = pair.split(":")
+    rng = dict(engines)[ename]
+    _, dt, call = [d for d in dists if d[0] == dname][0]
+    g = R.QRNG(rng) if "sobol" in ename else R.PRNG(rng, seed=12345)
+    g.stream = stream
+    b = torch.empty(n, dtype=dt, device="cuda")
+    ms = time_calls(lambda: call(g, b), stream, steps, warmup)
+    return {"pair": pair, "n": n, "steps": steps, "warmup": warmup, "ms_per_call": round(ms, 4),
+            "GB/s": round(_gbs(n * b.element_size(), ms), 1)}
+
+
+def per_generator_table(R, torch, stream, n, steps):
+    """GB/s for each engine x distribution at n samples (same workload as the step)."""
+    engines, dists = _pairs(R, torch)
     table = {}
     bufs = {}
     for ename, rng in engines:
@@ -237,6 +257,9 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain 2^28-float device copy (counter calibration)")
+    ap.add_argument("--only", metavar="ENGINE:KIND",
+                    help="time one pair of the per-generator table (e.g. philox4x32_10:normal_f32) and "
+                         "print its figures only; for profiler runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every generator here runs on a single MI355X and the "
@@ -250,6 +273,9 @@ def main():
     if args.calib:
         copy_calibration(torch, 1 << 28)
     stream = torch.cuda.Stream()
+    if args.only:
+        print(json.dumps(only_pair(R, torch, stream, n, args.only, args.steps, args.warmup)), flush=True)
+        return
     out = torch.empty(n, dtype=torch.float32, device="cuda")
     g = R.PRNG(R.PHILOX4_32_10, seed=12345)
     g.stream = stream
