@@ -4,7 +4,7 @@
  * Same names, state types and call semantics as the reference's device API
  * (library/include/rocrand/rocrand_kernel.h and the engine / distribution headers it
  * includes), for Philox4x32-10, MRG32k3a, MRG31k3p, XORWOW, LFSR113, ThreeFry
- * 2x32/2x64/4x32/4x64-20 and Sobol32/64 (plain and scrambled): rocrand_init, rocrand / rocrand2 / rocrand4, skipahead,
+ * 2x32/2x64/4x32/4x64-20, Sobol32/64 (plain and scrambled) and MTGP32: rocrand_init, rocrand / rocrand2 / rocrand4, skipahead,
  * skipahead_subsequence, skipahead_sequence, rocrand_uniform*, rocrand_normal*,
  * rocrand_log_normal*, rocrand_poisson* and rocrand_discrete*.  A state initialised with
  * (seed, subsequence, offset) produces the words of the reference state initialised the
@@ -14,7 +14,8 @@
  *
  * Sobol32/64 and their scrambled forms take the caller's direction vectors and scramble
  * constants (rocrand_get_direction_vectors32/64, rocrand_get_scramble_constants32/64).
- * Not provided here: the MTGP32 device state.
+ * MTGP32: block-shared states made by rocrand_make_state_mtgp32 from the parameter sets of
+ * rocrand_mtgp32_11213.h (the array is exported by librocrand.so).
  */
 #ifndef ROCRAND_KERNEL_H_
 #define ROCRAND_KERNEL_H_
@@ -22,12 +23,15 @@
 #include "rocrand_amd_engines.h"
 #include "rocrand_amd_jumptables.h"
 #include "rocrand_amd_math.h"
+#include "../rocrand.h"
 #include "../rocrand_discrete_types.h"
 
 #include <hip/hip_runtime.h>
 
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
 
 #define ROCRAND_PHILOX4x32_DEFAULT_SEED 0xdeadbeefdeadbeefULL
 #define ROCRAND_MRG32K3A_DEFAULT_SEED 12345ULL
@@ -37,6 +41,33 @@
 #define ROCRAND_LFSR113_DEFAULT_SEED_Y 8
 #define ROCRAND_LFSR113_DEFAULT_SEED_Z 16
 #define ROCRAND_LFSR113_DEFAULT_SEED_W 128
+
+// MTGP32 (rocrand_mtgp32.h:70-90): exponent 11213, a 351-word recurrence kept in a
+// 1024-word ring, 256 threads per block step, up to 512 parameter sets.
+#define MTGP_MEXP 11213
+#define MTGP_N 351
+#define MTGP_FLOOR_2P 256
+#define MTGP_CEIL_2P 512
+#define MTGP_TN MTGP_FLOOR_2P
+#define MTGP_LS (MTGP_TN * 3)
+#define MTGP_BN_MAX 512
+#define MTGP_TS 16
+#define MTGP_STATE 1024
+#define MTGP_MASK 1023
+
+// One parameter set as the MTGP dynamic creator writes it (rocrand_mtgp32.h:92-107).
+struct mtgp32_params_fast_t
+{
+    int           mexp;
+    int           pos;
+    int           sh1;
+    int           sh2;
+    uint32_t      tbl[16];
+    uint32_t      tmp_tbl[16];
+    uint32_t      flt_tmp_tbl[16];
+    uint32_t      mask;
+    unsigned char poly_sha1[21];
+};
 
 
 namespace rocrand_device
@@ -739,12 +770,238 @@ private:
 typedef scrambled_sobol_engine<unsigned int, 32>       scrambled_sobol32_engine;
 typedef scrambled_sobol_engine<unsigned long long, 64> scrambled_sobol64_engine;
 
+// ---------------------------------------------------------------- MTGP32
+// Parameter tables of up to MTGP_BN_MAX states in device memory (rocrand_make_constant).
+struct mtgp32_params
+{
+    unsigned int pos_tbl[MTGP_BN_MAX];
+    unsigned int param_tbl[MTGP_BN_MAX][MTGP_TS];
+    unsigned int temper_tbl[MTGP_BN_MAX][MTGP_TS];
+    unsigned int single_temper_tbl[MTGP_BN_MAX][MTGP_TS];
+    unsigned int sh1_tbl[MTGP_BN_MAX];
+    unsigned int sh2_tbl[MTGP_BN_MAX];
+    unsigned int mask[1];
+};
+typedef mtgp32_params_fast_t mtgp32_fast_params;
+
+// A ring of the recurrence: status[(offset + i) & MTGP_MASK] is word i of the current
+// window; id is the parameter set (block) it was made for.
+struct mtgp32_state
+{
+    int          offset;
+    int          id;
+    unsigned int status[MTGP_STATE];
+};
+
+// MTGP's seeding (rocrand_mtgp32.h:130-145): the first mexp/32 + 1 words from the seed,
+// the parameter set's hidden seed and Knuth's 1812433253 recurrence over a byte fill.
+inline void rocrand_mtgp32_init_state(unsigned int array[], const mtgp32_fast_params* para, unsigned int seed)
+{
+    const int          size   = para->mexp / 32 + 1;
+    const unsigned int hidden = para->tbl[4] ^ (para->tbl[8] << 16);
+    unsigned int       fill   = hidden;
+    fill += fill >> 16;
+    fill += fill >> 8;
+    memset(array, (int)(fill & 0xff), sizeof(unsigned int) * size);
+    array[0] = seed;
+    array[1] = hidden;
+    for(int i = 1; i < size; i++)
+        array[i] ^= 1812433253u * (array[i - 1] ^ (array[i - 1] >> 30)) + (unsigned int)i;
+}
+
+// One MTGP32 generator shared by the threads of a block: each call of next() by every
+// thread of the block advances the ring by blockDim.x words, thread t taking word t of
+// the step (rocrand_mtgp32.h:147-357).  The state must be visible to the whole block
+// (LDS, or one global copy per block) and every thread must make the same calls.
+class mtgp32_engine
+{
+public:
+    __host__ __device__ mtgp32_engine() {}
+    __host__ __device__ mtgp32_engine(const mtgp32_state& state, const mtgp32_params* params, int bid)
+    {
+        m_state = state;
+        load_params(params, bid);
+    }
+    // block-cooperative copy on the device (ends with a barrier), plain copy on the host
+    __host__ __device__ void copy(const mtgp32_engine* src)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        const unsigned int t = threadIdx.x;
+        for(unsigned int i = t; i < MTGP_STATE; i += blockDim.x)
+            m_state.status[i] = src->m_state.status[i];
+        if(t == 0)
+        {
+            m_state.offset = src->m_state.offset;
+            m_state.id     = src->m_state.id;
+            pos_tbl        = src->pos_tbl;
+            sh1_tbl        = src->sh1_tbl;
+            sh2_tbl        = src->sh2_tbl;
+            mask           = src->mask;
+        }
+        if(t < MTGP_TS)
+        {
+            param_tbl[t]         = src->param_tbl[t];
+            temper_tbl[t]        = src->temper_tbl[t];
+            single_temper_tbl[t] = src->single_temper_tbl[t];
+        }
+        __syncthreads();
+#else
+        *this = *src;
+#endif
+    }
+    __host__ __device__ void set_params(mtgp32_params* params) { load_params(params, m_state.id); }
+    __host__ __device__ unsigned int operator()() { return next(); }
+    // 32-bit output; the host has no block and returns 0 (as the reference)
+    __host__ __device__ unsigned int next()
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        const unsigned int t = threadIdx.x;
+        const unsigned int r = recur(t);
+        const unsigned int o = r ^ temper_tbl[fold(m_state.status[(t + m_state.offset + pos_tbl - 1) & MTGP_MASK])];
+        advance();
+        return o;
+#else
+        return 0;
+#endif
+    }
+    // the float-tempered output: bits of a float in [1, 2) shifted right by 9
+    __host__ __device__ unsigned int next_single()
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        const unsigned int t = threadIdx.x;
+        const unsigned int r = recur(t);
+        const unsigned int o
+            = (r >> 9) ^ single_temper_tbl[fold(m_state.status[(t + m_state.offset + pos_tbl - 1) & MTGP_MASK])];
+        advance();
+        return o;
+#else
+        return 0;
+#endif
+    }
+
+    mtgp32_state m_state;
+    unsigned int pos_tbl;
+    unsigned int param_tbl[MTGP_TS];
+    unsigned int temper_tbl[MTGP_TS];
+    unsigned int sh1_tbl;
+    unsigned int sh2_tbl;
+    unsigned int single_temper_tbl[MTGP_TS];
+    unsigned int mask;
+
+private:
+    __host__ __device__ void load_params(const mtgp32_params* params, int bid)
+    {
+        pos_tbl = params->pos_tbl[bid];
+        sh1_tbl = params->sh1_tbl[bid];
+        sh2_tbl = params->sh2_tbl[bid];
+        mask    = params->mask[0];
+        for(int j = 0; j < MTGP_TS; j++)
+        {
+            param_tbl[j]         = params->param_tbl[bid][j];
+            temper_tbl[j]        = params->temper_tbl[bid][j];
+            single_temper_tbl[j] = params->single_temper_tbl[bid][j];
+        }
+    }
+    // word MTGP_N of thread t's window, stored back into the ring.  A step reads ring
+    // words offset .. offset + pos + blockDim.x - 1 and writes offset + MTGP_N .., so with
+    // pos + blockDim.x <= MTGP_N (every parameter set here, blocks of up to 256) the
+    // threads of a step never read a word another thread writes.
+    __host__ __device__ unsigned int recur(unsigned int t)
+    {
+        const unsigned int o = m_state.offset;
+        unsigned int       x = (m_state.status[(t + o) & MTGP_MASK] & mask) ^ m_state.status[(t + o + 1) & MTGP_MASK];
+        x ^= x << sh1_tbl;
+        const unsigned int y = x ^ (m_state.status[(t + o + pos_tbl) & MTGP_MASK] >> sh2_tbl);
+        const unsigned int r = y ^ param_tbl[y & 0x0f];
+        m_state.status[(t + o + MTGP_N) & MTGP_MASK] = r;
+        return r;
+    }
+    __host__ __device__ static unsigned int fold(unsigned int v)
+    {
+        v ^= v >> 16;
+        v ^= v >> 8;
+        return v & 0x0f;
+    }
+#if defined(__HIP_DEVICE_COMPILE__)
+    __device__ void advance()
+    {
+        __syncthreads();
+        if(threadIdx.x == 0)
+            m_state.offset = (m_state.offset + (int)blockDim.x) & MTGP_MASK;
+        __syncthreads();
+    }
+#endif
+};
+
 } // namespace rocrand_device
 
 typedef rocrand_device::sobol32_engine           rocrand_state_sobol32;
 typedef rocrand_device::sobol64_engine           rocrand_state_sobol64;
 typedef rocrand_device::scrambled_sobol32_engine rocrand_state_scrambled_sobol32;
 typedef rocrand_device::scrambled_sobol64_engine rocrand_state_scrambled_sobol64;
+typedef rocrand_device::mtgp32_engine            rocrand_state_mtgp32;
+typedef rocrand_device::mtgp32_state             mtgp32_state;
+typedef rocrand_device::mtgp32_fast_params       mtgp32_fast_params;
+typedef rocrand_device::mtgp32_params            mtgp32_params;
+
+// n states for blocks 0 .. n-1 from parameter sets 0 .. n-1, copied to `state` (device
+// memory): seed ^ (seed >> 32) plus block + 1 seeds each (rocrand_mtgp32.h:362-398);
+// every state masks with the first set's mask.
+__host__ inline rocrand_status rocrand_make_state_mtgp32(rocrand_state_mtgp32*    state,
+                                                         const mtgp32_fast_params params[], int n,
+                                                         unsigned long long seed)
+{
+    if(n < 0 || n > MTGP_BN_MAX)
+        return ROCRAND_STATUS_OUT_OF_RANGE;
+    rocrand_state_mtgp32* h = (rocrand_state_mtgp32*)malloc(sizeof(rocrand_state_mtgp32) * (n > 0 ? n : 1));
+    if(h == NULL)
+        return ROCRAND_STATUS_ALLOCATION_FAILED;
+    seed ^= seed >> 32;
+    for(int i = 0; i < n; i++)
+    {
+        rocrand_device::rocrand_mtgp32_init_state(h[i].m_state.status, &params[i], (unsigned int)seed + i + 1);
+        h[i].m_state.offset = 0;
+        h[i].m_state.id     = i;
+        h[i].pos_tbl        = params[i].pos;
+        h[i].sh1_tbl        = params[i].sh1;
+        h[i].sh2_tbl        = params[i].sh2;
+        h[i].mask           = params[0].mask;
+        for(int j = 0; j < MTGP_TS; j++)
+        {
+            h[i].param_tbl[j]         = params[i].tbl[j];
+            h[i].temper_tbl[j]        = params[i].tmp_tbl[j];
+            h[i].single_temper_tbl[j] = params[i].flt_tmp_tbl[j];
+        }
+    }
+    const hipError_t e = hipMemcpy(state, h, sizeof(rocrand_state_mtgp32) * n, hipMemcpyDefault);
+    free(h);
+    return e == hipSuccess ? ROCRAND_STATUS_SUCCESS : ROCRAND_STATUS_ALLOCATION_FAILED;
+}
+
+// The parameter tables of all MTGP_BN_MAX sets into a device-resident mtgp32_params, for
+// rocrand_mtgp32_set_params (rocrand_mtgp32.h:400-470).
+__host__ inline rocrand_status rocrand_make_constant(const mtgp32_fast_params params[], mtgp32_params* p)
+{
+    mtgp32_params* h = (mtgp32_params*)malloc(sizeof(mtgp32_params));
+    if(h == NULL)
+        return ROCRAND_STATUS_ALLOCATION_FAILED;
+    h->mask[0] = params[0].mask;
+    for(int i = 0; i < MTGP_BN_MAX; i++)
+    {
+        h->pos_tbl[i] = params[i].pos;
+        h->sh1_tbl[i] = params[i].sh1;
+        h->sh2_tbl[i] = params[i].sh2;
+        for(int j = 0; j < MTGP_TS; j++)
+        {
+            h->param_tbl[i][j]         = params[i].tbl[j];
+            h->temper_tbl[i][j]        = params[i].tmp_tbl[j];
+            h->single_temper_tbl[i][j] = params[i].flt_tmp_tbl[j];
+        }
+    }
+    const hipError_t e = hipMemcpy(p, h, sizeof(mtgp32_params), hipMemcpyHostToDevice);
+    free(h);
+    return e == hipSuccess ? ROCRAND_STATUS_SUCCESS : ROCRAND_STATUS_ALLOCATION_FAILED;
+}
 typedef rocrand_device::philox4x32_10_engine   rocrand_state_philox4x32_10;
 typedef rocrand_device::mrg32k3a_engine        rocrand_state_mrg32k3a;
 typedef rocrand_device::mrg31k3p_engine        rocrand_state_mrg31k3p;
@@ -885,6 +1142,8 @@ __host__ __device__ __forceinline__ unsigned long long rocrand(rocrand_state_scr
 {
     return state->next();
 }
+// MTGP32: every thread of the block calls together (the generator steps per block)
+__device__ __forceinline__ unsigned int rocrand(rocrand_state_mtgp32* state) { return state->next(); }
 // the 32-bit states skip an unsigned int count (rocrand_sobol32.h skipahead -> discard)
 __host__ __device__ __forceinline__ void skipahead(unsigned long long offset, rocrand_state_sobol32* state)
 {
@@ -1346,6 +1605,22 @@ ROCRAND_AMD_ONE_WORD_NORMAL(rocrand_state_sobol64)
 ROCRAND_AMD_ONE_WORD_NORMAL(rocrand_state_scrambled_sobol32)
 ROCRAND_AMD_ONE_WORD_NORMAL(rocrand_state_scrambled_sobol64)
 #undef ROCRAND_AMD_ONE_WORD_NORMAL
+__device__ __forceinline__ float rocrand_uniform(rocrand_state_mtgp32* state)
+{
+    return rocrand_device::amd_detail::u_f32(rocrand(state));
+}
+__device__ __forceinline__ double rocrand_uniform_double(rocrand_state_mtgp32* state)
+{
+    return rocrand_device::amd_detail::u_f64(rocrand(state));
+}
+__device__ __forceinline__ float rocrand_normal(rocrand_state_mtgp32* state)
+{
+    return rocrand_device::amd_detail::normal_1(rocrand(state));
+}
+__device__ __forceinline__ double rocrand_normal_double(rocrand_state_mtgp32* state)
+{
+    return rocrand_device::amd_detail::normal_double_1(rocrand(state));
+}
 __host__ __device__ __forceinline__ float2 rocrand_normal2(rocrand_state_lfsr113* state)
 {
     const unsigned int a = rocrand(state);
@@ -1728,6 +2003,58 @@ __host__ __device__ __forceinline__ uint4 rocrand_discrete4(rocrand_state_philox
     return uint4{discrete_alias(draw(v.x), *discrete_distribution), discrete_alias(draw(v.y), *discrete_distribution),
                  discrete_alias(draw(v.z), *discrete_distribution),
                  discrete_alias(draw(v.w), *discrete_distribution)};
+}
+
+// ---------------------------------------------------------------- MTGP32 calls (device only)
+// Every thread of the block makes each call together (the generator steps per block).
+__device__ __forceinline__ void rocrand_mtgp32_block_copy(rocrand_state_mtgp32* src, rocrand_state_mtgp32* dest)
+{
+    dest->copy(src);
+}
+__device__ __forceinline__ void rocrand_mtgp32_set_params(rocrand_state_mtgp32* state, mtgp32_params* params)
+{
+    state->set_params(params);
+}
+__device__ __forceinline__ float2 rocrand_normal2(rocrand_state_mtgp32* state)
+{
+    const unsigned int a = rocrand(state);
+    const unsigned int b = rocrand(state);
+    return rocrand_device::amd_detail::box_muller(a, b);
+}
+__device__ __forceinline__ double2 rocrand_normal_double2(rocrand_state_mtgp32* state)
+{
+    return rocrand_device::amd_detail::bmd2_words(state);
+}
+__device__ __forceinline__ float rocrand_log_normal(rocrand_state_mtgp32* state, float mean, float stddev)
+{
+    return rocrand_device::amd_detail::ln_f(mean, stddev, rocrand_normal(state));
+}
+__device__ __forceinline__ float2 rocrand_log_normal2(rocrand_state_mtgp32* state, float mean, float stddev)
+{
+    using namespace rocrand_device::amd_detail;
+    const float2 r = rocrand_normal2(state);
+    return float2{ln_f(mean, stddev, r.x), ln_f(mean, stddev, r.y)};
+}
+__device__ __forceinline__ double rocrand_log_normal_double(rocrand_state_mtgp32* state, double mean, double stddev)
+{
+    return rocrand_device::amd_detail::ln_d(mean, stddev, rocrand_normal_double(state));
+}
+__device__ __forceinline__ double2 rocrand_log_normal_double2(rocrand_state_mtgp32* state, double mean,
+                                                              double stddev)
+{
+    using namespace rocrand_device::amd_detail;
+    const double2 r = rocrand_normal_double2(state);
+    return double2{ln_d(mean, stddev, r.x), ln_d(mean, stddev, r.y)};
+}
+__device__ __forceinline__ unsigned int rocrand_poisson(rocrand_state_mtgp32* state, double lambda)
+{
+    return rocrand_device::amd_detail::poisson_inv(state, lambda);
+}
+__device__ __forceinline__ unsigned int rocrand_discrete(rocrand_state_mtgp32*               state,
+                                                         const rocrand_discrete_distribution discrete_distribution)
+{
+    using namespace rocrand_device::amd_detail;
+    return discrete_alias(draw(rocrand(state)), *discrete_distribution);
 }
 
 #undef ROCRAND_AMD_BM_FLOAT

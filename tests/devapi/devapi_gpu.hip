@@ -117,3 +117,83 @@ extern "C" int devapi_gpu_run(int engine, int fn, unsigned long long seed, const
         return -3;
     return 0;
 }
+
+// ---------------------------------------------------------------- MTGP32
+namespace
+{
+
+template<int FN>
+__global__ __launch_bounds__(256) void devapi_mtgp32_kernel(rocrand_state_mtgp32* states, mtgp32_params* params,
+                                                            int calls, double a, double b,
+                                                            rocrand_discrete_distribution dist,
+                                                            devapi_test::sobol_tables tables, uint64_t* out)
+{
+    __shared__ rocrand_state_mtgp32 state;
+    rocrand_mtgp32_block_copy(&states[blockIdx.x], &state);
+    if(params != nullptr)
+    {
+        if(threadIdx.x == 0)
+            rocrand_mtgp32_set_params(&state, params);
+        __syncthreads();
+    }
+    uint64_t* o = out + ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * calls * 4;
+    for(int k = 0; k < calls; k++)
+        devapi_test::call<rocrand_state_mtgp32, FN>(&state, k, a, b, dist, o + (size_t)k * 4);
+    (void)tables;
+}
+
+} // namespace
+
+// MTGP32 states 0 .. blocks-1 from `params` (host array of mtgp32_fast_params), one block of
+// `threads` threads per state; with use_constant the kernels also reload the tables from a
+// rocrand_make_constant copy.  out: [blocks][threads][calls][4] slots.
+extern "C" int devapi_gpu_mtgp32(int fn, unsigned long long seed, int blocks, int threads, int calls, double a,
+                                 double b, unsigned int dsize, const double* prob, const unsigned int* alias,
+                                 const void* params, int use_constant, unsigned long long* out)
+{
+    if(blocks <= 0 || blocks > MTGP_BN_MAX || threads <= 0 || threads > 256 || calls <= 0 || calls > 4096
+       || !params || dsize == 0 || !prob || !alias)
+        return -2;
+    if(!devapi_test::supported<rocrand_state_mtgp32>(fn))
+        return -1;
+    device_buffers     buf;
+    const size_t       out_bytes = (size_t)blocks * threads * calls * 4 * sizeof(uint64_t);
+    if(hipMalloc(&buf.p[0], sizeof(rocrand_state_mtgp32) * blocks) != hipSuccess
+       || hipMalloc(&buf.p[1], sizeof(mtgp32_params)) != hipSuccess || hipMalloc(&buf.p[2], out_bytes) != hipSuccess
+       || hipMalloc(&buf.p[3], dsize * sizeof(double)) != hipSuccess
+       || hipMalloc(&buf.p[4], dsize * sizeof(unsigned int)) != hipSuccess
+       || hipMalloc(&buf.p[5], sizeof(rocrand_discrete_distribution_st)) != hipSuccess)
+        return -3;
+    const mtgp32_fast_params* fp = static_cast<const mtgp32_fast_params*>(params);
+    if(rocrand_make_state_mtgp32((rocrand_state_mtgp32*)buf.p[0], fp, blocks, seed) != ROCRAND_STATUS_SUCCESS
+       || rocrand_make_constant(fp, (mtgp32_params*)buf.p[1]) != ROCRAND_STATUS_SUCCESS)
+        return -3;
+    rocrand_discrete_distribution_st d;
+    d.size        = dsize;
+    d.offset      = 0;
+    d.alias       = (unsigned int*)buf.p[4];
+    d.probability = (double*)buf.p[3];
+    d.cdf         = nullptr;
+    if(hipMemcpy(buf.p[3], prob, dsize * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
+       || hipMemcpy(buf.p[4], alias, dsize * sizeof(unsigned int), hipMemcpyHostToDevice) != hipSuccess
+       || hipMemcpy(buf.p[5], &d, sizeof(d), hipMemcpyHostToDevice) != hipSuccess
+       || hipMemset(buf.p[2], 0, out_bytes) != hipSuccess)
+        return -3;
+    const devapi_test::sobol_tables none = {nullptr, nullptr, 1};
+    devapi_test::with_fn(fn,
+                         [&](auto fc)
+                         {
+                             constexpr int FN = decltype(fc)::value;
+                             if constexpr(devapi_test::supported<rocrand_state_mtgp32>(FN))
+                                 hipLaunchKernelGGL((devapi_mtgp32_kernel<FN>), dim3(blocks), dim3(threads), 0, 0,
+                                                    (rocrand_state_mtgp32*)buf.p[0],
+                                                    use_constant ? (mtgp32_params*)buf.p[1] : nullptr, calls, a, b,
+                                                    (rocrand_discrete_distribution)buf.p[5], none,
+                                                    (uint64_t*)buf.p[2]);
+                         });
+    if(hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        return -4;
+    if(hipMemcpy(out, buf.p[2], out_bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        return -3;
+    return 0;
+}

@@ -178,3 +178,26 @@ def test_gpu_driver_exports():
 
     lib = ctypes.CDLL(C.GPU_LIB)
     assert hasattr(lib, "devapi_gpu_run")
+
+
+def test_exported_mtgp32_parameter_sets():
+    """mtgp32dc_params_fast_11213 (librocrand.so, declared in rocrand_mtgp32_11213.h): 512
+    mtgp32_params_fast_t records of exponent 11213 holding the library's own parameter
+    sets, with the float-tempering table 0x3f800000 | tmp_tbl >> 9."""
+    import ctypes
+
+    from rocrand_amd import rocrand as R
+
+    L = R.load()
+    addr = ctypes.addressof(ctypes.c_char.in_dll(L, "mtgp32dc_params_fast_11213"))
+    raw = np.frombuffer(ctypes.string_at(addr, 512 * 236), dtype=np.uint8).reshape(512, 236)
+    words = raw[:, :212].copy().view("<u4")
+    ours = np.fromfile(os.path.join(os.path.dirname(HERE), "rocrand_amd", "data", "mtgp32_11213.bin"),
+                       dtype="<u4").reshape(512, 36)
+    assert np.all(words[:, 0] == 11213)
+    assert np.array_equal(words[:, 1:4], ours[:, 0:3])          # pos, sh1, sh2
+    assert np.array_equal(words[:, 4:20], ours[:, 3:19])        # tbl
+    assert np.array_equal(words[:, 20:36], ours[:, 19:35])      # tmp_tbl
+    assert np.array_equal(words[:, 36:52], 0x3F800000 | (ours[:, 19:35] >> 9))
+    assert np.array_equal(words[:, 52], ours[:, 35])            # mask
+    assert np.all(raw[:, 233:] == 0) and np.all(raw[:, 212:233].any(axis=1))

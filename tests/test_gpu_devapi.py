@@ -197,3 +197,79 @@ def test_sobol_states_equal_the_bulk_quasi_generator(gpu, engine, rng, bits):
     got = gpu.run(engine, C.F["WORD"], sub, off, n, vectors=vectors)[:, :, 0]
     want = bulk.cpu().numpy().reshape(dims, n).astype(np.uint64)
     assert np.array_equal(got, want)
+
+
+def _mtgp32(fn, seed, calls, use_constant=0, blocks=512, threads=256):
+    import ctypes
+
+    from rocrand_amd import rocrand as R
+
+    L = R.load()
+    params = ctypes.addressof(ctypes.c_char.in_dll(L, "mtgp32dc_params_fast_11213"))
+    lib = ctypes.CDLL(C.GPU_LIB)
+    f = lib.devapi_gpu_mtgp32
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                  ctypes.c_double, ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                  ctypes.c_void_p]
+    prob, alias, _, _ = C.discrete_table()
+    out = np.zeros(blocks * threads * calls * 4, dtype=np.uint64)
+    rc = f(fn, seed, blocks, threads, calls, C.MEAN, C.SD, len(prob), prob.ctypes.data, alias.ctypes.data, params,
+           use_constant, out.ctypes.data)
+    assert rc == 0, rc
+    return out.reshape(blocks, threads, calls, 4)
+
+
+@pytest.mark.parametrize("use_constant", [0, 1])
+def test_mtgp32_states_equal_the_bulk_generator(gpu, use_constant):
+    """States from rocrand_make_state_mtgp32(mtgp32dc_params_fast_11213, 512, seed), one
+    256-thread block each (state in LDS via rocrand_mtgp32_block_copy; optionally tables
+    reloaded by rocrand_mtgp32_set_params from rocrand_make_constant): thread t of block b
+    makes the words the bulk MTGP32 generator writes at vec b*256 + t + j*131072 (the
+    reference's mtgp32.hpp output map; one word per u32/uniform vec, a Box-Muller pair of
+    two consecutive steps per normal vec)."""
+    import torch
+
+    from rocrand_amd import rocrand as R
+
+    seed, K = 0x0123456789ABCDEF, 8
+    n = 512 * 256 * K
+    vec = np.arange(512)[:, None, None] * 256 + np.arange(256)[None, :, None]
+    idx = vec + np.arange(K)[None, None, :] * 131072
+    idx2 = ((vec + (np.arange(K)[None, None, :] // 2) * 131072) * 2 + np.arange(K)[None, None, :] % 2)
+    for kind, fn, dt in (("u32", "WORD", torch.uint32), ("uniform", "UNIFORM", torch.float32),
+                         ("normal", "NORMAL2", torch.float32)):
+        g = R.PRNG(R.MTGP32, seed=seed)
+        bulk = torch.empty(n, dtype=dt, device="cuda")
+        if kind == "u32":
+            g.generate(bulk)
+        elif kind == "uniform":
+            g.uniform(bulk)
+        else:
+            g.normal(bulk, 0.0, 1.0)
+        torch.cuda.synchronize()
+        want = bulk.cpu().numpy().view(np.uint32)
+        print(f"case mtgp32 {kind} use_constant={use_constant}", flush=True)
+        if fn == "NORMAL2":
+            got = _mtgp32(C.F[fn], seed, K // 2, use_constant)[:, :, :, :2].reshape(512, 256, K)
+        else:
+            got = _mtgp32(C.F[fn], seed, K, use_constant)[:, :, :, 0]
+        assert np.array_equal(got.astype(np.uint32), want[idx2 if fn == "NORMAL2" else idx]), kind
+
+
+def test_mtgp32_calls_match_their_word_transforms(gpu):
+    """Poisson, discrete and the one-word normals of an MTGP32 state are the shared
+    transforms of the words the same state yields (pinned for the other states)."""
+    seed, K = 99, 12
+    words = _mtgp32(C.F["WORD"], seed, K, blocks=4)[:, :, :, 0].astype(np.uint32)
+    u = _mtgp32(C.F["UNIFORM_D"], seed, K, blocks=4)[:, :, :, 0].view(np.float64)
+    assert np.array_equal(u, 2.0**-32 + words.astype(np.float64) * 2.0**-32)
+    disc = _mtgp32(C.F["DISCRETE"], seed, K, blocks=4)[:, :, :, 0].astype(np.uint32)
+    prob, alias, _, _ = C.discrete_table()  # the MTGP32 driver's table has offset 0
+    x = words.astype(np.float64) * 2.0**-32 * len(prob)
+    i = np.floor(x).astype(np.int64)
+    want = np.where(x - i < prob[i], i, alias[i]).astype(np.uint32)
+    assert np.array_equal(disc, want)
+    pois = _mtgp32(C.F["POISSON"], seed, 3, blocks=4)[:, :, :, 0].astype(np.float64)
+    lam = C.MEAN
+    assert abs(pois.mean() - lam) < 5 * np.sqrt(lam / pois.size)
