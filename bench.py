@@ -92,7 +92,8 @@ def per_generator_table(R, torch, stream, n, steps):
             if dt not in bufs:
                 bufs[dt] = torch.empty(n, dtype=dt, device="cuda")
             b = bufs[dt]
-            ms = time_calls(lambda: call(g, b), stream, steps, 2)
+            # clocks settle per kernel: ALU-heavy transforms run slower for their first calls
+            ms = time_calls(lambda: call(g, b), stream, steps, 40)
             row[dname] = round(_gbs(n * b.element_size(), ms), 1)
         table[ename] = row
         print(f"# {ename}: {row}", file=sys.stderr, flush=True)
@@ -301,7 +302,7 @@ def main():
     value = _gbs(nbytes, ms)
 
     ceiling = store_ceiling(torch, stream, n, 10)
-    table = None if args.no_table else per_generator_table(R, torch, stream, n, 5)
+    table = None if args.no_table else per_generator_table(R, torch, stream, n, 10)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
     batched = None if args.no_batched else batched_bench(R, torch)
     if batched:
