@@ -35,3 +35,23 @@ extern "C" int ref_devapi_run(int engine, int fn, unsigned long long seed, const
         return -2;
     return devapi_test::run_host(engine, fn, seed, sub, off, n, calls, a, b, &d, tables, (uint64_t*)out);
 }
+
+// Bytes of the reference's state object after rocrand_init, as 32-bit words.
+extern "C" int ref_devapi_state(int engine, unsigned long long seed, const unsigned long long* sub,
+                                const unsigned long long* off, int n, int words, unsigned int* out)
+{
+    int size = -1;
+    const devapi_test::sobol_tables none = {nullptr, nullptr, 1};
+    if(engine >= devapi_test::SOBOL32)
+        return -1;
+    devapi_test::with_engine(engine,
+                             [&](auto* tag)
+                             {
+                                 using S = std::remove_pointer_t<decltype(tag)>;
+                                 for(int t = 0; t < n; t++)
+                                     devapi_test::state_words<S>(seed, sub[t], off[t], none, out + (size_t)t * words,
+                                                                 words);
+                                 size = (int)sizeof(S);
+                             });
+    return size;
+}

@@ -136,3 +136,64 @@ def host():
 
 def gpu():
     return Driver(GPU_LIB, "devapi_gpu_run")
+
+
+# 32-bit words of each state that rocrand_init defines (the Box-Muller cache values and
+# padding are left unset by init, as in the reference)
+STATE_WORDS = {
+    "philox": list(range(13)),            # counter, result, key, substate, cache flags
+    "mrg32k3a": list(range(8)),           # g1, g2, cache flags
+    "mrg31k3p": list(range(8)),
+    "xorwow": [0, 1, 2] + list(range(6, 11)),  # d, cache flags, x[5]
+    "lfsr113": list(range(8)),            # z, subsequence seed
+    "threefry2x32": list(range(7)),       # counter, key, result, substate
+    "threefry2x64": list(range(13)),
+    "threefry4x32": list(range(13)),
+    "threefry4x64": list(range(25)),
+}
+
+
+def _bind_state(lib, name):
+    f = getattr(lib, name)
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                  ctypes.c_void_p]
+    return f
+
+
+def state_words(path, symbol, engine, seed, sub, off, words=32):
+    """[threads, words] uint32 state bytes after rocrand_init, and sizeof(state)."""
+    f = _bind_state(ctypes.CDLL(path), symbol)
+    sub = np.ascontiguousarray(sub, dtype=np.uint64)
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    out = np.zeros(len(sub) * words, dtype=np.uint32)
+    size = f(E[engine], seed, sub.ctypes.data, off.ctypes.data, len(sub), words, out.ctypes.data)
+    assert size > 0, size
+    return out.reshape(len(sub), words), size
+
+
+# The reference's device-API state checks (test_rocrand_kernel_philox4x32_10.cpp:227-271,
+# test_rocrand_kernel_xorwow.cpp:211-250).
+def philox_kat(words):
+    n = words.shape[0]
+    assert np.all(words[:, 8] == 0xBEEFDEAD) and np.all(words[:, 9] == 0xDEADBEEF)     # key
+    assert np.all(words[:, 0] == 0) and np.all(words[:, 1] == 17)                     # counter.x, .y
+    assert np.array_equal(words[:, 2], np.arange(n, dtype=np.uint32))                  # counter.z = subsequence
+    assert np.all(words[:, 3] == 0)
+    assert np.all(words[:, 4:8].any(axis=1))                                          # result set
+    assert np.all(words[:, 10] == 0)                                                  # substate
+
+
+def xorwow_kat(words, seed, offset):
+    x = words[:, 6:11]
+    assert np.all(x[1:] != x[:-1])
+    assert np.all(words[:, 0] == words[0, 0])
+    # d = 6615241 + t1 + t0 + 362437 * offset (mod 2^32), rocrand_xorwow.h:100-145
+    s0 = (seed & 0xFFFFFFFF) ^ 0x2C7F967F
+    s1 = (seed >> 32) ^ 0xA03697CB
+    d = (6615241 + (2073658381 * s1) + (1228688033 * s0) + 362437 * (offset & 0xFFFFFFFF)) & 0xFFFFFFFF
+    assert int(words[0, 0]) == d
+
+
+KAT_SEED = 0xDEADBEEFBEEFDEAD
+PHILOX_KAT_OFFSET = 4 * ((0xFFFFFFFF * 17) + 17)

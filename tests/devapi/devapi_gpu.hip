@@ -197,3 +197,51 @@ extern "C" int devapi_gpu_mtgp32(int fn, unsigned long long seed, int blocks, in
         return -3;
     return 0;
 }
+
+// ---------------------------------------------------------------- state bytes after init
+namespace
+{
+template<class S>
+__global__ __launch_bounds__(BLOCK) void devapi_state_kernel(unsigned long long seed, const unsigned long long* sub,
+                                                             const unsigned long long* off, int n, int words,
+                                                             uint32_t* out)
+{
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if(t >= n)
+        return;
+    const devapi_test::sobol_tables none = {nullptr, nullptr, 1};
+    devapi_test::state_words<S>(seed, sub[t], off[t], none, out + (size_t)t * words, words);
+}
+} // namespace
+
+extern "C" int devapi_gpu_state(int engine, unsigned long long seed, const unsigned long long* sub,
+                                const unsigned long long* off, int n, int words, unsigned int* out)
+{
+    if(n <= 0 || n > (1 << 20) || words <= 0 || words > 64 || engine >= devapi_test::SOBOL32)
+        return -2;
+    device_buffers buf;
+    const size_t   out_bytes = (size_t)n * words * sizeof(uint32_t);
+    if(hipMalloc(&buf.p[0], n * sizeof(unsigned long long)) != hipSuccess
+       || hipMalloc(&buf.p[1], n * sizeof(unsigned long long)) != hipSuccess
+       || hipMalloc(&buf.p[2], out_bytes) != hipSuccess)
+        return -3;
+    if(hipMemcpy(buf.p[0], sub, n * sizeof(unsigned long long), hipMemcpyHostToDevice) != hipSuccess
+       || hipMemcpy(buf.p[1], off, n * sizeof(unsigned long long), hipMemcpyHostToDevice) != hipSuccess)
+        return -3;
+    int                size   = -1;
+    const unsigned int blocks = (unsigned int)((n + BLOCK - 1) / BLOCK);
+    devapi_test::with_engine(engine,
+                             [&](auto* tag)
+                             {
+                                 using S = std::remove_pointer_t<decltype(tag)>;
+                                 hipLaunchKernelGGL((devapi_state_kernel<S>), dim3(blocks), dim3(BLOCK), 0, 0, seed,
+                                                    (const unsigned long long*)buf.p[0],
+                                                    (const unsigned long long*)buf.p[1], n, words, (uint32_t*)buf.p[2]);
+                                 size = (int)sizeof(S);
+                             });
+    if(hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        return -4;
+    if(hipMemcpy(out, buf.p[2], out_bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        return -3;
+    return size;
+}

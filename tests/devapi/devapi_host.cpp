@@ -23,3 +23,24 @@ extern "C" int devapi_host_run(int engine, int fn, unsigned long long seed, cons
         return -2;
     return devapi_test::run_host(engine, fn, seed, sub, off, n, calls, a, b, &d, tables, (uint64_t*)out);
 }
+
+// State bytes after rocrand_init for thread t = (seed, sub[t], off[t]), `words` 32-bit
+// words per thread; returns sizeof(state) or -1.
+extern "C" int devapi_host_state(int engine, unsigned long long seed, const unsigned long long* sub,
+                                 const unsigned long long* off, int n, int words, unsigned int* out)
+{
+    int size = -1;
+    const devapi_test::sobol_tables none = {nullptr, nullptr, 1};
+    if(engine >= devapi_test::SOBOL32)
+        return -1;
+    devapi_test::with_engine(engine,
+                             [&](auto* tag)
+                             {
+                                 using S = std::remove_pointer_t<decltype(tag)>;
+                                 for(int t = 0; t < n; t++)
+                                     devapi_test::state_words<S>(seed, sub[t], off[t], none, out + (size_t)t * words,
+                                                                 words);
+                                 size = (int)sizeof(S);
+                             });
+    return size;
+}

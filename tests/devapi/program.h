@@ -363,6 +363,21 @@ __host__ __device__ inline void run_thread(unsigned long long seed, unsigned lon
         call<S, FN>(&s, k, a, b, dist, out + (size_t)k * 4);
 }
 
+// The raw bytes of a state after init(seed, sub, off), as 32-bit words (zero-padded).
+template<class S>
+__host__ __device__ inline void state_words(unsigned long long seed, unsigned long long sub, unsigned long long off,
+                                            const sobol_tables& tables, uint32_t* out, int words)
+{
+    S s;
+    memset(&s, 0, sizeof(S));
+    init(&s, seed, sub, off, tables);
+    const int n = (int)((sizeof(S) + 3) / 4);
+    uint32_t  w[(sizeof(S) + 3) / 4] = {};
+    memcpy(w, &s, sizeof(S));
+    for(int i = 0; i < words; i++)
+        out[i] = i < n ? w[i] : 0u;
+}
+
 // Calls fn(std::integral_constant<int, FN>) for the runtime fn id.
 template<class F>
 inline bool with_fn(int fn, F&& f)

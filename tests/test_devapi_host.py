@@ -201,3 +201,29 @@ def test_exported_mtgp32_parameter_sets():
     assert np.array_equal(words[:, 36:52], 0x3F800000 | (ours[:, 19:35] >> 9))
     assert np.array_equal(words[:, 52], ours[:, 35])            # mask
     assert np.all(raw[:, 233:] == 0) and np.all(raw[:, 212:233].any(axis=1))
+
+
+def test_reference_state_kats_host(host):
+    """The reference's device-API state checks (test_rocrand_kernel_philox4x32_10.cpp:227-271,
+    test_rocrand_kernel_xorwow.cpp:211-250) on the host branch."""
+    sub = np.arange(256, dtype=np.uint64)
+    w, size = C.state_words(C.HOST_LIB, "devapi_host_state", "philox", C.KAT_SEED, sub,
+                            np.full(256, C.PHILOX_KAT_OFFSET, dtype=np.uint64))
+    assert size == 64
+    C.philox_kat(w)
+    w, _ = C.state_words(C.HOST_LIB, "devapi_host_state", "xorwow", C.KAT_SEED, sub, np.full(256, 345678, np.uint64))
+    C.xorwow_kat(w, C.KAT_SEED, 345678)
+
+
+@pytest.mark.parametrize("engine", list(C.STATE_WORDS))
+def test_state_layout_and_values_equal_reference(host, engine):
+    """The state objects hold the reference's fields in the reference's order and size:
+    after rocrand_init the defined words equal the reference state's bytes."""
+    if C.reference() is None:
+        pytest.skip("oracle/_ref/libref.so not built (needs the reference sources)")
+    sub, off = C.thread_params(64, seed=9)
+    ours, size = C.state_words(C.HOST_LIB, "devapi_host_state", engine, 0x0123456789ABCDEF, sub, off)
+    ref, ref_size = C.state_words(C.REF_LIB, "ref_devapi_state", engine, 0x0123456789ABCDEF, sub, off)
+    assert size == ref_size
+    cols = C.STATE_WORDS[engine]
+    assert np.array_equal(ours[:, cols], ref[:, cols])

@@ -273,3 +273,25 @@ def test_mtgp32_calls_match_their_word_transforms(gpu):
     pois = _mtgp32(C.F["POISSON"], seed, 3, blocks=4)[:, :, :, 0].astype(np.float64)
     lam = C.MEAN
     assert abs(pois.mean() - lam) < 5 * np.sqrt(lam / pois.size)
+
+
+def test_gpu_reference_state_kats(gpu):
+    """The reference's device-API state checks (test_rocrand_kernel_philox4x32_10.cpp:227-271,
+    test_rocrand_kernel_xorwow.cpp:211-250), on states initialised in a kernel."""
+    sub = np.arange(256, dtype=np.uint64)
+    w, size = C.state_words(C.GPU_LIB, "devapi_gpu_state", "philox", C.KAT_SEED, sub,
+                            np.full(256, C.PHILOX_KAT_OFFSET, dtype=np.uint64))
+    assert size == 64
+    C.philox_kat(w)
+    w, _ = C.state_words(C.GPU_LIB, "devapi_gpu_state", "xorwow", C.KAT_SEED, sub, np.full(256, 345678, np.uint64))
+    C.xorwow_kat(w, C.KAT_SEED, 345678)
+
+
+@pytest.mark.parametrize("engine", list(C.STATE_WORDS))
+def test_gpu_states_equal_host_states(gpu, engine):
+    sub, off = C.thread_params(4096, seed=17)
+    g, size = C.state_words(C.GPU_LIB, "devapi_gpu_state", engine, 0xFEEDFACE12345678, sub, off)
+    h, hsize = C.state_words(C.HOST_LIB, "devapi_host_state", engine, 0xFEEDFACE12345678, sub, off)
+    assert size == hsize
+    cols = C.STATE_WORDS[engine]
+    assert np.array_equal(g[:, cols], h[:, cols])
