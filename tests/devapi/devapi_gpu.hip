@@ -245,3 +245,129 @@ extern "C" int devapi_gpu_state(int engine, unsigned long long seed, const unsig
         return -3;
     return size;
 }
+
+// ---------------------------------------------------------------- throughput of user kernels
+// A user kernel as the device API is used: one state per thread (rocrand_init with the
+// thread as subsequence), then a grid-stride loop of generate calls writing coalesced
+// 16-byte vectors (4 values per thread per iteration).  kind: 0 raw words, 1 uniform f32,
+// 2 normal f32.
+namespace
+{
+
+template<class S>
+__global__ __launch_bounds__(256) void bench_init_kernel(S* states, unsigned long long seed)
+{
+    const unsigned int                 t    = blockIdx.x * 256 + threadIdx.x;
+    const devapi_test::sobol_tables    none = {nullptr, nullptr, 1};
+    S                                  s;
+    devapi_test::init(&s, seed, t, 0, none);
+    states[t] = s;
+}
+
+template<class S>
+__device__ __forceinline__ uint4 bench4(S* s, int kind)
+{
+    if(kind == 0)
+    {
+        if constexpr(devapi_test::has_rocrand4<S>::value)
+            return rocrand4(s);
+        else
+        {
+            const unsigned int a = rocrand(s), b = rocrand(s), c = rocrand(s), d = rocrand(s);
+            return uint4{a, b, c, d};
+        }
+    }
+    float4 f;
+    if(kind == 1)
+    {
+        if constexpr(devapi_test::has_uniform4<S>::value)
+            f = rocrand_uniform4(s);
+        else
+        {
+            f.x = rocrand_uniform(s);
+            f.y = rocrand_uniform(s);
+            f.z = rocrand_uniform(s);
+            f.w = rocrand_uniform(s);
+        }
+    }
+    else
+    {
+        if constexpr(devapi_test::has_normal4<S>::value)
+            f = rocrand_normal4(s);
+        else
+        {
+            const float2 a = rocrand_normal2(s), b = rocrand_normal2(s);
+            f              = float4{a.x, a.y, b.x, b.y};
+        }
+    }
+    return uint4{__float_as_uint(f.x), __float_as_uint(f.y), __float_as_uint(f.z), __float_as_uint(f.w)};
+}
+
+template<class S>
+__global__ __launch_bounds__(256) void bench_gen_kernel(S* states, uint4* out, unsigned int iters, int kind)
+{
+    const unsigned int t       = blockIdx.x * 256 + threadIdx.x;
+    const unsigned int threads = gridDim.x * 256;
+    S                  s       = states[t];
+    for(unsigned int i = 0; i < iters; i++)
+        out[(size_t)i * threads + t] = bench4(&s, kind);
+    states[t] = s;
+}
+
+} // namespace
+
+// Times `reps` generate launches of 2^28 values (after 20 untimed) and one init launch;
+// writes ms per generate launch and ms of the init into ms[0], ms[1].
+extern "C" int devapi_gpu_bench(int engine, int kind, int reps, float* ms)
+{
+    if(kind < 0 || kind > 2 || reps <= 0 || reps > 1000
+       || !(engine == devapi_test::PHILOX || engine == devapi_test::MRG32K3A || engine == devapi_test::MRG31K3P
+            || engine == devapi_test::XORWOW || engine == devapi_test::LFSR113 || engine == devapi_test::TF4X32))
+        return -2;
+    constexpr unsigned int blocks = 2048, threads = blocks * 256;
+    constexpr size_t       n4     = (size_t)1 << 26; // 2^28 values as uint4
+    const unsigned int     iters  = (unsigned int)(n4 / threads);
+    device_buffers         buf;
+    if(hipMalloc(&buf.p[0], n4 * sizeof(uint4)) != hipSuccess || hipMalloc(&buf.p[1], (size_t)threads * 64) != hipSuccess)
+        return -3;
+    hipEvent_t e0, e1;
+    if(hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return -3;
+    int  rc  = 0;
+    auto run = [&](auto* tag)
+        {
+            using S = std::remove_pointer_t<decltype(tag)>;
+            static_assert(sizeof(S) <= 64, "state slot");
+            S* st = (S*)buf.p[1];
+            (void)hipEventRecord(e0, 0);
+            hipLaunchKernelGGL((bench_init_kernel<S>), dim3(blocks), dim3(256), 0, 0, st, 12345ull);
+            (void)hipEventRecord(e1, 0);
+            if(hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms[1], e0, e1) != hipSuccess)
+                rc = -4;
+            for(int r = 0; r < 20; r++)
+                hipLaunchKernelGGL((bench_gen_kernel<S>), dim3(blocks), dim3(256), 0, 0, st, (uint4*)buf.p[0], iters,
+                                   kind);
+            (void)hipEventRecord(e0, 0);
+            for(int r = 0; r < reps; r++)
+                hipLaunchKernelGGL((bench_gen_kernel<S>), dim3(blocks), dim3(256), 0, 0, st, (uint4*)buf.p[0], iters,
+                                   kind);
+            (void)hipEventRecord(e1, 0);
+            if(hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms[0], e0, e1) != hipSuccess)
+                rc = -4;
+            ms[0] /= reps;
+        };
+    switch(engine)
+    {
+        case devapi_test::PHILOX: run((rocrand_state_philox4x32_10*)nullptr); break;
+        case devapi_test::MRG32K3A: run((rocrand_state_mrg32k3a*)nullptr); break;
+        case devapi_test::MRG31K3P: run((rocrand_state_mrg31k3p*)nullptr); break;
+        case devapi_test::XORWOW: run((rocrand_state_xorwow*)nullptr); break;
+        case devapi_test::LFSR113: run((rocrand_state_lfsr113*)nullptr); break;
+        default: run((rocrand_state_threefry4x32_20*)nullptr); break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if(hipGetLastError() != hipSuccess)
+        return -4;
+    return rc;
+}

@@ -295,3 +295,23 @@ def test_gpu_states_equal_host_states(gpu, engine):
     assert size == hsize
     cols = C.STATE_WORDS[engine]
     assert np.array_equal(g[:, cols], h[:, cols])
+
+
+def test_device_api_throughput(gpu):
+    """A user kernel over the device API (one state per thread, grid-stride loop of 16-byte
+    stores, 2^28 values per launch): GB/s per engine and kind, printed for the record."""
+    import ctypes
+
+    lib = ctypes.CDLL(C.GPU_LIB)
+    f = lib.devapi_gpu_bench
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    ms = (ctypes.c_float * 2)()
+    for engine in ("philox", "threefry4x32", "mrg32k3a", "mrg31k3p", "xorwow", "lfsr113"):
+        row = []
+        for kind, name in enumerate(("u32", "uniform_f32", "normal_f32")):
+            assert f(C.E[engine], kind, 50, ms) == 0
+            gbs = (1 << 28) * 4 / (ms[0] * 1e-3) / 1e9
+            row.append(f"{name} {gbs:.0f} GB/s")
+            assert gbs > 100
+        print(f"device API {engine}: {', '.join(row)}; init of 524288 states {ms[1]:.3f} ms", flush=True)
