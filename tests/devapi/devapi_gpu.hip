@@ -250,7 +250,7 @@ extern "C" int devapi_gpu_state(int engine, unsigned long long seed, const unsig
 // A user kernel as the device API is used: one state per thread (rocrand_init with the
 // thread as subsequence), then a grid-stride loop of generate calls writing coalesced
 // 16-byte vectors (4 values per thread per iteration).  kind: 0 raw words, 1 uniform f32,
-// 2 normal f32.
+// 2 normal f32, 3 no generation (the same loop storing a constant: its own ceiling).
 namespace
 {
 
@@ -264,10 +264,15 @@ __global__ __launch_bounds__(256) void bench_init_kernel(S* states, unsigned lon
     states[t] = s;
 }
 
-template<class S>
-__device__ __forceinline__ uint4 bench4(S* s, int kind)
+template<class S, int KIND>
+__device__ __forceinline__ uint4 bench4(S* s)
 {
-    if(kind == 0)
+    if constexpr(KIND == 3)
+    {
+        // the loop's own ceiling: a value that costs nothing to make
+        return uint4{threadIdx.x, blockIdx.x, 0u, 1u};
+    }
+    else if constexpr(KIND == 0)
     {
         if constexpr(devapi_test::has_rocrand4<S>::value)
             return rocrand4(s);
@@ -278,7 +283,7 @@ __device__ __forceinline__ uint4 bench4(S* s, int kind)
         }
     }
     float4 f;
-    if(kind == 1)
+    if constexpr(KIND == 1)
     {
         if constexpr(devapi_test::has_uniform4<S>::value)
             f = rocrand_uniform4(s);
@@ -303,14 +308,14 @@ __device__ __forceinline__ uint4 bench4(S* s, int kind)
     return uint4{__float_as_uint(f.x), __float_as_uint(f.y), __float_as_uint(f.z), __float_as_uint(f.w)};
 }
 
-template<class S>
-__global__ __launch_bounds__(256) void bench_gen_kernel(S* states, uint4* out, unsigned int iters, int kind)
+template<class S, int KIND>
+__global__ __launch_bounds__(256) void bench_gen_kernel(S* states, uint4* out, unsigned int iters)
 {
     const unsigned int t       = blockIdx.x * 256 + threadIdx.x;
     const unsigned int threads = gridDim.x * 256;
     S                  s       = states[t];
     for(unsigned int i = 0; i < iters; i++)
-        out[(size_t)i * threads + t] = bench4(&s, kind);
+        out[(size_t)i * threads + t] = bench4<S, KIND>(&s);
     states[t] = s;
 }
 
@@ -320,7 +325,7 @@ __global__ __launch_bounds__(256) void bench_gen_kernel(S* states, uint4* out, u
 // writes ms per generate launch and ms of the init into ms[0], ms[1].
 extern "C" int devapi_gpu_bench(int engine, int kind, int reps, float* ms)
 {
-    if(kind < 0 || kind > 2 || reps <= 0 || reps > 1000
+    if(kind < 0 || kind > 3 || reps <= 0 || reps > 1000
        || !(engine == devapi_test::PHILOX || engine == devapi_test::MRG32K3A || engine == devapi_test::MRG31K3P
             || engine == devapi_test::XORWOW || engine == devapi_test::LFSR113 || engine == devapi_test::TF4X32))
         return -2;
@@ -344,13 +349,26 @@ extern "C" int devapi_gpu_bench(int engine, int kind, int reps, float* ms)
             (void)hipEventRecord(e1, 0);
             if(hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms[1], e0, e1) != hipSuccess)
                 rc = -4;
+            auto gen = [&]()
+            {
+                if(kind == 0)
+                    hipLaunchKernelGGL((bench_gen_kernel<S, 0>), dim3(blocks), dim3(256), 0, 0, st, (uint4*)buf.p[0],
+                                       iters);
+                else if(kind == 1)
+                    hipLaunchKernelGGL((bench_gen_kernel<S, 1>), dim3(blocks), dim3(256), 0, 0, st, (uint4*)buf.p[0],
+                                       iters);
+                else if(kind == 3)
+                    hipLaunchKernelGGL((bench_gen_kernel<S, 3>), dim3(blocks), dim3(256), 0, 0, st, (uint4*)buf.p[0],
+                                       iters);
+                else
+                    hipLaunchKernelGGL((bench_gen_kernel<S, 2>), dim3(blocks), dim3(256), 0, 0, st, (uint4*)buf.p[0],
+                                       iters);
+            };
             for(int r = 0; r < 20; r++)
-                hipLaunchKernelGGL((bench_gen_kernel<S>), dim3(blocks), dim3(256), 0, 0, st, (uint4*)buf.p[0], iters,
-                                   kind);
+                gen();
             (void)hipEventRecord(e0, 0);
             for(int r = 0; r < reps; r++)
-                hipLaunchKernelGGL((bench_gen_kernel<S>), dim3(blocks), dim3(256), 0, 0, st, (uint4*)buf.p[0], iters,
-                                   kind);
+                gen();
             (void)hipEventRecord(e1, 0);
             if(hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms[0], e0, e1) != hipSuccess)
                 rc = -4;

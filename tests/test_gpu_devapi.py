@@ -307,6 +307,8 @@ def test_device_api_throughput(gpu):
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
     ms = (ctypes.c_float * 2)()
+    assert f(C.E["philox"], 3, 50, ms) == 0
+    print(f"device API loop without generation: {(1 << 28) * 4 / (ms[0] * 1e-3) / 1e9:.0f} GB/s", flush=True)
     for engine in ("philox", "threefry4x32", "mrg32k3a", "mrg31k3p", "xorwow", "lfsr113"):
         row = []
         for kind, name in enumerate(("u32", "uniform_f32", "normal_f32")):
