@@ -326,6 +326,58 @@ hipError_t run_groups(const std::vector<group>& groups, const device_tables& tab
     return hipSuccess;
 }
 
+// Streams and events of a fork/join capture: one side stream per group.
+struct fork_join
+{
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t>  done;
+    hipEvent_t               fork = nullptr;
+
+    bool create(size_t n)
+    {
+        if(hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess)
+            return false;
+        side.assign(n, nullptr);
+        done.assign(n, nullptr);
+        for(size_t i = 0; i < n; i++)
+            if(hipStreamCreateWithFlags(&side[i], hipStreamNonBlocking) != hipSuccess
+               || hipEventCreateWithFlags(&done[i], hipEventDisableTiming) != hipSuccess)
+                return false;
+        return true;
+    }
+    ~fork_join()
+    {
+        for(hipStream_t q : side)
+            if(q)
+                (void)hipStreamDestroy(q);
+        for(hipEvent_t e : done)
+            if(e)
+                (void)hipEventDestroy(e);
+        if(fork)
+            (void)hipEventDestroy(fork);
+    }
+};
+
+// run_groups under capture, each group on its side stream between fork and join.
+hipError_t run_groups_forked(const std::vector<group>& groups, const device_tables& tables, hipStream_t cap,
+                             const fork_join& fj)
+{
+    hipError_t e = hipEventRecord(fj.fork, cap);
+    for(size_t i = 0; i < groups.size() && e == hipSuccess; i++)
+    {
+        const group& g = groups[i];
+        if(g.tiles.empty())
+            continue;
+        const uint32_t blocks = std::min<uint32_t>(g.blocks, (uint32_t)g.tiles.size());
+        if((e = hipStreamWaitEvent(fj.side[i], fj.fork, 0)) != hipSuccess
+           || (e = g.fn(g.d_calls, g.d_tiles, (uint32_t)g.tiles.size(), blocks, tables, fj.side[i])) != hipSuccess
+           || (e = hipEventRecord(fj.done[i], fj.side[i])) != hipSuccess
+           || (e = hipStreamWaitEvent(cap, fj.done[i], 0)) != hipSuccess)
+            break;
+    }
+    return e;
+}
+
 // Device blocks with pinned host twins for the one-shot call's descriptor images,
 // recycled once the call that used them has completed (its event has fired).  Blocks
 // come in power-of-two sizes from 64 KiB and stay allocated for the process lifetime,
@@ -686,11 +738,18 @@ rocrand_status ROCRANDAPI rocrand_batch_plan_create(rocrand_batch_plan*         
                 st = ROCRAND_STATUS_INTERNAL_ERROR;
             if(st == ROCRAND_STATUS_SUCCESS && cap != stream && hipStreamSynchronize(stream) != hipSuccess)
                 st = ROCRAND_STATUS_INTERNAL_ERROR;
+            // Each group is captured on its own stream forked from and joined back to the
+            // capture stream, so the graph's kernel nodes are independent: groups bound by
+            // arithmetic (normals, Poisson) and groups bound by stores (uniforms) may run
+            // side by side and one group's tail overlaps the next.
+            fork_join fj;
+            if(st == ROCRAND_STATUS_SUCCESS && !fj.create(p->groups.size()))
+                st = ROCRAND_STATUS_INTERNAL_ERROR;
             if(st == ROCRAND_STATUS_SUCCESS && hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed) != hipSuccess)
                 st = ROCRAND_STATUS_INTERNAL_ERROR;
             else if(st == ROCRAND_STATUS_SUCCESS)
             {
-                const hipError_t le = run_groups(p->groups, p->tables, cap);
+                const hipError_t le = run_groups_forked(p->groups, p->tables, cap, fj);
                 const hipError_t ce = hipStreamEndCapture(cap, &p->graph);
                 if(le != hipSuccess || ce != hipSuccess
                    || hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) != hipSuccess)
