@@ -31,7 +31,21 @@ __global__ __launch_bounds__(256) void counter_fast_kernel(counter_call c)
 {
     constexpr uint32_t TILE = SUB ? 63u : 64u;
     const size_t       wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if(wave * TILE < c.vec_n)
+    if constexpr(dist_kind<D>::value == out_kind::poisson)
+    {
+        // the packed alias table in LDS (dynamic, size * 8 bytes): one LDS read per sample
+        // instead of a global load with a per-lane address
+        extern __shared__ uint64_t lds_packed[];
+        const uint32_t             size = c.params.huge ? 0u : c.params.pt.size; // no table above 4000
+        for(uint32_t i = threadIdx.x; i < size; i += 256)
+            lds_packed[i] = c.params.pt.packed[i];
+        __syncthreads();
+        counter_call cl   = c;
+        cl.params.pt.packed = lds_packed;
+        if(wave * TILE < c.vec_n)
+            fast_tile<K, D, SUB>(cl, wave * TILE, c.vec_n);
+    }
+    else if(wave * TILE < c.vec_n)
         fast_tile<K, D, SUB>(c, wave * TILE, c.vec_n);
     if(blockIdx.x == 0 && threadIdx.x == 0)
         head_tail<K, D>(c);
@@ -54,10 +68,13 @@ hipError_t launch_one(const counter_call& c, hipStream_t s)
                 blocks = 1;
             if(blocks > 0x7fffffffu)
                 return hipErrorInvalidConfiguration;
+            const size_t lds = dist_kind<D>::value == out_kind::poisson && !c.params.huge
+                                   ? c.params.pt.size * sizeof(uint64_t)
+                                   : 0;
             if(c.vec_base.sub)
-                hipLaunchKernelGGL((counter_fast_kernel<K, D, true>), dim3((unsigned)blocks), dim3(256), 0, s, c);
+                hipLaunchKernelGGL((counter_fast_kernel<K, D, true>), dim3((unsigned)blocks), dim3(256), lds, s, c);
             else
-                hipLaunchKernelGGL((counter_fast_kernel<K, D, false>), dim3((unsigned)blocks), dim3(256), 0, s, c);
+                hipLaunchKernelGGL((counter_fast_kernel<K, D, false>), dim3((unsigned)blocks), dim3(256), lds, s, c);
             return hipGetLastError();
         }
     }
