@@ -44,6 +44,29 @@ __device__ __forceinline__ void store_bytes(void* dst, const void* src)
         *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(src);
 }
 
+// Poisson: each block copies the packed alias table into dynamic LDS (size * 8 bytes, no
+// table above lambda 4000) and samples from there.  Every thread of the block calls this
+// before any thread leaves.
+template<class D>
+__device__ __forceinline__ void stage_poisson_table(interleaved_call& c)
+{
+    if constexpr(dist_kind<D>::value == out_kind::poisson)
+    {
+        extern __shared__ uint64_t lds_packed[];
+        const uint32_t             size = c.params.huge ? 0u : c.params.pt.size;
+        for(uint32_t i = threadIdx.x; i < size; i += blockDim.x)
+            lds_packed[i] = c.params.pt.packed[i];
+        __syncthreads();
+        c.params.pt.packed = lds_packed;
+    }
+}
+
+template<class D>
+size_t poisson_lds_bytes(const interleaved_call& c)
+{
+    return dist_kind<D>::value == out_kind::poisson && !c.params.huge ? c.params.pt.size * sizeof(uint64_t) : 0;
+}
+
 template<inter_kind K, class D>
 __global__ __launch_bounds__(256) void generate_kernel(interleaved_call c)
 {
@@ -53,6 +76,7 @@ __global__ __launch_bounds__(256) void generate_kernel(interleaved_call c)
     constexpr int BYTES  = OUT * (int)sizeof(out_t);
     static_assert(BYTES == 4 || BYTES == 8 || BYTES == 16, "vec size");
 
+    stage_poisson_table<D>(c);
     const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if(id >= S)
         return;
@@ -137,7 +161,8 @@ __global__ __launch_bounds__(256) void generate_chunked_kernel(interleaved_call 
     const uint32_t     j   = blockIdx.x / BPC;
     const uint32_t     id  = (blockIdx.x % BPC) * 256 + threadIdx.x;
     if(j >= J)
-        return;
+        return; // the whole block
+    stage_poisson_table<D>(c);
     // vec-steps of this engine: vec indices id, id + S, ... below vec_n
     const uint64_t steps  = id < c.vec_n ? (c.vec_n - id + S - 1) / S : 0;
     const uint64_t k0     = (uint64_t)j * C;
@@ -235,8 +260,8 @@ hipError_t launch_gen(const interleaved_call& c, hipStream_t s)
                 hipError_t     err = get_device_tables(t);
                 if(err != hipSuccess)
                     return err;
-                hipLaunchKernelGGL((generate_chunked_kernel<K, D>), dim3(S / 256 * J), dim3(256), 0, s, c, t, J,
-                                   (uint32_t)C);
+                hipLaunchKernelGGL((generate_chunked_kernel<K, D>), dim3(S / 256 * J), dim3(256),
+                                   poisson_lds_bytes<D>(c), s, c, t, J, (uint32_t)C);
                 return hipGetLastError();
             }
         }
@@ -255,11 +280,12 @@ hipError_t launch_gen(const interleaved_call& c, hipStream_t s)
             if(err != hipSuccess)
                 return err;
             const uint32_t C = (uint32_t)((kmax + J - 1) / J);
-            hipLaunchKernelGGL((generate_chunked_kernel<K, D>), dim3(S / 256 * J), dim3(256), 0, s, c, t, J, C);
+            hipLaunchKernelGGL((generate_chunked_kernel<K, D>), dim3(S / 256 * J), dim3(256),
+                               poisson_lds_bytes<D>(c), s, c, t, J, C);
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((generate_kernel<K, D>), dim3(S / 256), dim3(256), 0, s, c);
+    hipLaunchKernelGGL((generate_kernel<K, D>), dim3(S / 256), dim3(256), poisson_lds_bytes<D>(c), s, c);
     return hipGetLastError();
 }
 
