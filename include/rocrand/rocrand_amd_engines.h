@@ -339,24 +339,34 @@ __host__ __device__ __forceinline__ uint32_t mrg32_reduce_m2(uint64_t p)
     return (uint32_t)(p >= MRG32_M2 ? p - MRG32_M2 : p);
 }
 
-// x mod m for x < 2m + 2^32 and m = 2^32 - c, from the 64-bit value's words: x - m is
-// lo + c (mod 2^32) exactly when x >= m, i.e. when hi != 0 or lo >= m.
-// x mod m for x < 2m, m = 2^32 - c, in three 32-bit operations and no lane masks:
-// v = x mod 2^32 + [x >= 2^32] c is x - m when x >= 2^32 (then v < m) and x otherwise;
+// (a mod 2^24) * (b mod 2^24) mod 2^32: one v_mad_u32_u24 together with an addition.
+__host__ __device__ __forceinline__ uint32_t umul24(uint32_t a, uint32_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul24(a, b);
+#else
+    return (a & 0xffffffu) * (b & 0xffffffu);
+#endif
+}
+
+// x mod m for m = 2^32 - c and x < 2^33 - c (so hi = x >> 32 is 0 or 1), in three
+// 32-bit operations and no lane masks: v = lo + c·hi cannot wrap and is x - m·hi;
 // v + c wraps to v - m exactly when v >= m, so the smaller of v and v + c (mod 2^32)
 // is the residue.
 __host__ __device__ __forceinline__ uint32_t mrg32_fold_final(uint64_t x, uint32_t c)
 {
     const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-    const uint32_t v  = lo + ((0u - hi) & c);
+    const uint32_t v  = lo + umul24(hi, c);
     const uint32_t w  = v + c;
     return w < v ? w : v;
 }
 
 __host__ __device__ __forceinline__ uint32_t mrg32k3a_next(mrg_state& s)
 {
-    // products below 2^53; one fold by 2^32 = 209 (mod m1) leaves x < 2 m1, two folds by
-    // 2^32 = 22853 (mod m2) leave x < 2 m2: the same residues as the reference's reductions
+    // t1 < (a12 + a13n) m1 < 2214308 * 2^32, so one fold by 2^32 = 209 (mod m1) leaves
+    // x < 2^32 + 462790164; t2 < (a21 + a23n) m2 < 1898201 * 2^32, so two folds by
+    // 2^32 = 22853 (mod m2) leave x < 2^32 + 251384: both within mrg32_fold_final's
+    // range, and the same residues as the reference's reductions
     const uint64_t t1 = (uint64_t)(uint32_t)MRG32_A12 * s.g1[1]
                         + (uint64_t)(uint32_t)MRG32_A13N * ((uint32_t)MRG32_M1 - s.g1[0]);
     const uint32_t p1 = mrg32_fold_final(209ull * (t1 >> 32) + (uint32_t)t1, 209u);

@@ -81,18 +81,26 @@ struct seq_reader
     }
 };
 
+// Non-temporal vector stores (global_store_dwordx2/x4 ... nt): the output is written
+// once and not read back by the kernel.
 template<int BYTES>
 __device__ __forceinline__ void store_vec(void* dst, const uint32_t* u)
 {
+    using v2 = unsigned int __attribute__((ext_vector_type(2)));
+    using v4 = unsigned int __attribute__((ext_vector_type(4)));
     if constexpr(BYTES == 8)
     {
-        *reinterpret_cast<uint2*>(dst) = make_uint2(u[0], u[1]);
+        const v2 t = {u[0], u[1]};
+        __builtin_nontemporal_store(t, reinterpret_cast<v2*>(dst));
     }
     else
     {
 #pragma unroll
         for(int j = 0; j < BYTES / 16; j++)
-            reinterpret_cast<uint4*>(dst)[j] = make_uint4(u[4 * j], u[4 * j + 1], u[4 * j + 2], u[4 * j + 3]);
+        {
+            const v4 t = {u[4 * j], u[4 * j + 1], u[4 * j + 2], u[4 * j + 3]};
+            __builtin_nontemporal_store(t, reinterpret_cast<v4*>(dst) + j);
+        }
     }
 }
 

@@ -44,6 +44,26 @@ __device__ __forceinline__ void store_bytes(void* dst, const void* src)
         *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(src);
 }
 
+// Non-temporal store of BYTES at base + off through a raw buffer resource built from a
+// wavefront-uniform base: the address arithmetic stays in scalar registers (a global
+// store with a per-lane 64-bit pointer costs two VALU additions a step).  `limit` bounds
+// the byte offsets (the hardware drops a store at or beyond it).
+template<int BYTES>
+__device__ __forceinline__ void store_uniform_base(char* base, uint32_t off, uint32_t limit, const void* src)
+{
+    // dword 3 of a gfx9 buffer resource: 32-bit data format, raw (stride 0) addressing
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(base, 0, (int)limit, 0x00020000);
+    constexpr int NT = 2; // cache-policy bit 1: nt on gfx950
+    using v2 = unsigned int __attribute__((ext_vector_type(2)));
+    using v4 = unsigned int __attribute__((ext_vector_type(4)));
+    if constexpr(BYTES == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(*reinterpret_cast<const uint32_t*>(src), r, (int)off, 0, NT);
+    else if constexpr(BYTES == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(*reinterpret_cast<const v2*>(src), r, (int)off, 0, NT);
+    else
+        __builtin_amdgcn_raw_buffer_store_b128(*reinterpret_cast<const v4*>(src), r, (int)off, 0, NT);
+}
+
 // Poisson: each block copies the packed alias table into dynamic LDS (size * 8 bytes, no
 // table above lambda 4000) and samples from there.  Every thread of the block calls this
 // before any thread leaves.
@@ -193,23 +213,38 @@ __global__ __launch_bounds__(256) void generate_chunked_kernel(interleaved_call 
     out_t* const vec_data = data + c.mis;
     alignas(16) out_t o[OUT];
     uint32_t     in[IN];
-    const uint64_t k1 = ending ? steps : (k0 + C < steps ? k0 + C : steps);
-    // 32-bit loop state; three steps per iteration turn the state rotation into renaming
-    const uint32_t cnt  = (uint32_t)(k1 - k0);
-    size_t         index = id + k0 * S;
+    const uint64_t k1  = ending ? steps : (k0 + C < steps ? k0 + C : steps);
+    const uint32_t cnt = (uint32_t)(k1 - k0);
+    // Steps that every working lane of the block takes: the block's last engine has the
+    // fewest vec-steps (steps_lo), and every lane's count is min(C, steps_lo - k0) or one
+    // more, so that many run under a scalar loop counter and the rest (at most one) per
+    // lane.  Stores go to a block-uniform base (scalar registers, advanced by S vecs a
+    // step) plus the lane's 32-bit byte offset.
+    const uint32_t id_last  = (blockIdx.x % BPC) * 256 + 255;
+    const uint64_t steps_lo = id_last < c.vec_n ? (c.vec_n - id_last + S - 1) / S : 0;
+    const uint32_t ucnt
+        = steps_lo > k0 ? (uint32_t)(steps_lo - k0 < C ? steps_lo - k0 : (uint64_t)C) : 0u;
+    char*          base = reinterpret_cast<char*>(vec_data + k0 * S * OUT);
+    const uint32_t off  = id * (uint32_t)BYTES;
+    const uint32_t lim  = S * (uint32_t)BYTES;
+    uint32_t       k    = 0;
+    // three steps per iteration turn the state rotation into renaming
 #pragma unroll 3
-    for(uint32_t k = 0; k < cnt; k++, index += S)
+    for(; k < ucnt; k++, base += (size_t)S * BYTES)
     {
 #pragma unroll
         for(int i = 0; i < IN; i++)
             in[i] = eng.next();
         D::apply(in, o, c.params);
-        if constexpr(BYTES == 4)
-            *reinterpret_cast<uint32_t*>(vec_data + index * OUT) = *reinterpret_cast<const uint32_t*>(o);
-        else if constexpr(BYTES == 8)
-            *reinterpret_cast<uint2*>(vec_data + index * OUT) = *reinterpret_cast<const uint2*>(o);
-        else
-            *reinterpret_cast<uint4*>(vec_data + index * OUT) = *reinterpret_cast<const uint4*>(o);
+        store_uniform_base<BYTES>(base, off, lim, o);
+    }
+    for(; k < cnt; k++, base += (size_t)S * BYTES)
+    {
+#pragma unroll
+        for(int i = 0; i < IN; i++)
+            in[i] = eng.next();
+        D::apply(in, o, c.params);
+        store_uniform_base<BYTES>(base, off, lim, o);
     }
     if(!ending)
         return;
