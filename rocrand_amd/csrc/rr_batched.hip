@@ -34,7 +34,7 @@ namespace rr
 namespace
 {
 
-constexpr uint32_t CTR_TILE_VECS = 1024; // 4 wavefronts x 4 chunks of 64
+constexpr uint32_t CTR_TILE_VECS = 4096; // 4 wavefronts x 16 chunks of 64
 constexpr uint32_t MRG_TILE      = 256;
 
 struct batch_tile
@@ -58,17 +58,43 @@ struct inter_call
     dist_params params;
 };
 
+// Poisson: the block samples from its copy of the tile's alias table in dynamic LDS
+// (sized for the group's largest table), copied again only when a tile refers to another
+// table than the one staged last (tables are shared per lambda within a plan).  The
+// tile, and so `p`, is block-uniform; every thread of the block calls this.
+template<class D>
+__device__ __forceinline__ void stage_tile_table(dist_params& p, const uint64_t*& staged)
+{
+    if constexpr(dist_kind<D>::value == out_kind::poisson)
+    {
+        extern __shared__ uint64_t lds_packed[];
+        if(p.huge)
+            return;
+        if(p.pt.packed != staged)
+        {
+            __syncthreads(); // no wavefront still samples the previous table
+            for(uint32_t i = threadIdx.x; i < p.pt.size; i += blockDim.x)
+                lds_packed[i] = p.pt.packed[i];
+            __syncthreads();
+            staged = p.pt.packed;
+        }
+        p.pt.packed = lds_packed;
+    }
+}
+
 template<ctr_kind K, class D>
 __global__ __launch_bounds__(256) void batch_counter_kernel(const counter_call* calls,
                                                             const batch_tile*   tiles,
                                                             uint32_t            ntiles)
 {
     using namespace ctr_dev;
+    const uint64_t* staged = nullptr;
     for(uint32_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x)
     {
-        const batch_tile   tl  = tiles[ti];
-        const counter_call c   = calls[tl.stream];
-        const size_t       end = tl.first + tl.count;
+        const batch_tile tl = tiles[ti];
+        counter_call     c  = calls[tl.stream];
+        stage_tile_table<D>(c.params, staged);
+        const size_t end = tl.first + tl.count;
         if(fast_ok<K>(c))
         {
             const uint32_t T    = c.vec_base.sub ? 63u : 64u;
@@ -102,14 +128,16 @@ __global__ __launch_bounds__(256) void batch_inter_kernel(const inter_call* call
     constexpr int IN    = D::in_w;
     constexpr int OUT   = D::out_w;
     constexpr int BYTES = OUT * (int)sizeof(out_t);
+    const uint64_t* staged = nullptr;
     for(uint32_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x)
     {
         const batch_tile tl = tiles[ti];
         const uint32_t   id = (uint32_t)tl.first + threadIdx.x;
+        inter_call       c  = calls[tl.stream];
+        stage_tile_table<D>(c.params, staged);
         if(threadIdx.x >= tl.count)
             continue;
-        const inter_call& c   = calls[tl.stream];
-        const uint32_t    eid = (id + c.start) % S;
+        const uint32_t eid = (id + c.start) % S;
         ieng<K>           g;
         g.set(c.base[eid < c.start ? 1 : 0]);
         g.sub(t, eid);
@@ -117,19 +145,30 @@ __global__ __launch_bounds__(256) void batch_inter_kernel(const inter_call* call
         out_t* const vec_data = data + c.mis;
         alignas(16) out_t o[OUT];
         uint32_t     in[IN];
-        size_t       index = id;
-        for(; index < c.vec_n; index += S)
+        // The tile's last engine takes the fewest vec-steps: that many run for every lane
+        // under a scalar counter, storing from a tile-uniform base (buffer stores), and the
+        // rest (at most one) per lane.
+        const uint32_t     id_last = (uint32_t)tl.first + tl.count - 1;
+        const uint32_t     ucnt    = id_last < c.vec_n ? (uint32_t)((c.vec_n - id_last + S - 1) / S) : 0u;
+        const uint32_t     off     = id * (uint32_t)BYTES;
+        constexpr uint32_t lim     = S * (uint32_t)BYTES;
+        char*              base    = reinterpret_cast<char*>(vec_data);
+        size_t             index   = id;
+        for(uint32_t k = 0; k < ucnt; k++, index += S, base += (size_t)S * BYTES)
         {
 #pragma unroll
             for(int i = 0; i < IN; i++)
                 in[i] = g.next();
             D::apply(in, o, c.params);
-            if constexpr(BYTES == 4)
-                *reinterpret_cast<uint32_t*>(vec_data + index * OUT) = *reinterpret_cast<const uint32_t*>(o);
-            else if constexpr(BYTES == 8)
-                *reinterpret_cast<uint2*>(vec_data + index * OUT) = *reinterpret_cast<const uint2*>(o);
-            else
-                *reinterpret_cast<uint4*>(vec_data + index * OUT) = *reinterpret_cast<const uint4*>(o);
+            store_uniform_base<BYTES>(base, off, lim, o);
+        }
+        for(; index < c.vec_n; index += S, base += (size_t)S * BYTES)
+        {
+#pragma unroll
+            for(int i = 0; i < IN; i++)
+                in[i] = g.next();
+            D::apply(in, o, c.params);
+            store_uniform_base<BYTES>(base, off, lim, o);
         }
         // the engine that would produce vec index vec_n writes the head, then the tail
         // (mrg.hpp:114-153)
@@ -174,23 +213,24 @@ out_kind kind_of(rocrand_batch_kind k)
     }
 }
 
+// `lds` = bytes of dynamic LDS for the group's largest Poisson table (0 otherwise).
 using launcher = hipError_t (*)(const void* calls, const batch_tile* tiles, uint32_t ntiles, uint32_t blocks,
-                                const device_tables& t, hipStream_t s);
+                                uint32_t lds, const device_tables& t, hipStream_t s);
 
 template<ctr_kind K, class D>
 hipError_t launch_counter_group(const void* calls, const batch_tile* tiles, uint32_t ntiles, uint32_t blocks,
-                                const device_tables&, hipStream_t s)
+                                uint32_t lds, const device_tables&, hipStream_t s)
 {
-    hipLaunchKernelGGL((batch_counter_kernel<K, D>), dim3(blocks), dim3(256), 0, s,
+    hipLaunchKernelGGL((batch_counter_kernel<K, D>), dim3(blocks), dim3(256), lds, s,
                        static_cast<const counter_call*>(calls), tiles, ntiles);
     return hipGetLastError();
 }
 
 template<inter_kind K, class D>
 hipError_t launch_inter_group(const void* calls, const batch_tile* tiles, uint32_t ntiles, uint32_t blocks,
-                              const device_tables& t, hipStream_t s)
+                              uint32_t lds, const device_tables& t, hipStream_t s)
 {
-    hipLaunchKernelGGL((batch_inter_kernel<K, D>), dim3(blocks), dim3(256), 0, s,
+    hipLaunchKernelGGL((batch_inter_kernel<K, D>), dim3(blocks), dim3(256), lds, s,
                        static_cast<const inter_call*>(calls), tiles, ntiles, t);
     return hipGetLastError();
 }
@@ -232,6 +272,7 @@ struct group
     void*                     d_calls   = nullptr;
     batch_tile*               d_tiles   = nullptr;
     uint32_t                  blocks    = 2048;
+    uint32_t                  lds_bytes = 0; // largest Poisson table of the group, packed
 };
 
 // Grid sizes chosen per (engine, kind, log2 of the group's output bytes).
@@ -319,7 +360,7 @@ hipError_t run_groups(const std::vector<group>& groups, const device_tables& tab
         if(g.tiles.empty())
             continue;
         const uint32_t blocks = std::min<uint32_t>(g.blocks, (uint32_t)g.tiles.size());
-        hipError_t     e      = g.fn(g.d_calls, g.d_tiles, (uint32_t)g.tiles.size(), blocks, tables, s);
+        hipError_t     e      = g.fn(g.d_calls, g.d_tiles, (uint32_t)g.tiles.size(), blocks, g.lds_bytes, tables, s);
         if(e != hipSuccess)
             return e;
     }
@@ -370,7 +411,7 @@ hipError_t run_groups_forked(const std::vector<group>& groups, const device_tabl
             continue;
         const uint32_t blocks = std::min<uint32_t>(g.blocks, (uint32_t)g.tiles.size());
         if((e = hipStreamWaitEvent(fj.side[i], fj.fork, 0)) != hipSuccess
-           || (e = g.fn(g.d_calls, g.d_tiles, (uint32_t)g.tiles.size(), blocks, tables, fj.side[i])) != hipSuccess
+           || (e = g.fn(g.d_calls, g.d_tiles, (uint32_t)g.tiles.size(), blocks, g.lds_bytes, tables, fj.side[i])) != hipSuccess
            || (e = hipEventRecord(fj.done[i], fj.side[i])) != hipSuccess
            || (e = hipStreamWaitEvent(cap, fj.done[i], 0)) != hipSuccess)
             break;
@@ -565,6 +606,8 @@ rocrand_status build(batch_layout& L, const rocrand_batch_stream* streams, size_
             git = gid.emplace(key, L.groups.size() - 1).first;
         }
         group&         g  = L.groups[git->second];
+        if(k == out_kind::poisson && !prm.huge)
+            g.lds_bytes = std::max<uint32_t>(g.lds_bytes, prm.pt.size * (uint32_t)sizeof(uint64_t));
         const uint32_t si = (uint32_t)(philox ? g.ccalls.size() : g.icalls.size());
         dist_shape     s{};
         if(philox)
@@ -666,7 +709,7 @@ rocrand_status tune(rocrand_batch_plan_st& p)
             {
                 if(rep == 1 && hipEventRecord(a, p.stream) != hipSuccess)
                     return ROCRAND_STATUS_INTERNAL_ERROR;
-                if(g.fn(g.d_calls, g.d_tiles, (uint32_t)g.tiles.size(), blocks, p.tables, p.stream) != hipSuccess)
+                if(g.fn(g.d_calls, g.d_tiles, (uint32_t)g.tiles.size(), blocks, g.lds_bytes, p.tables, p.stream) != hipSuccess)
                     return ROCRAND_STATUS_LAUNCH_FAILURE;
             }
             float ms = 0.f;

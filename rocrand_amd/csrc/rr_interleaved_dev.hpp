@@ -244,5 +244,25 @@ struct ieng<inter_kind::lfsr113>
 };
 
 
+// Non-temporal store of BYTES at base + off through a raw buffer resource built from a
+// wavefront-uniform base: the address arithmetic stays in scalar registers (a global
+// store with a per-lane 64-bit pointer costs two VALU additions a step).  `limit` bounds
+// the byte offsets (the hardware drops a store at or beyond it).
+template<int BYTES>
+__device__ __forceinline__ void store_uniform_base(char* base, uint32_t off, uint32_t limit, const void* src)
+{
+    // dword 3 of a gfx9 buffer resource: 32-bit data format, raw (stride 0) addressing
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(base, 0, (int)limit, 0x00020000);
+    constexpr int NT = 2; // cache-policy bit 1: nt on gfx950
+    using v2 = unsigned int __attribute__((ext_vector_type(2)));
+    using v4 = unsigned int __attribute__((ext_vector_type(4)));
+    if constexpr(BYTES == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(*reinterpret_cast<const uint32_t*>(src), r, (int)off, 0, NT);
+    else if constexpr(BYTES == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(*reinterpret_cast<const v2*>(src), r, (int)off, 0, NT);
+    else
+        __builtin_amdgcn_raw_buffer_store_b128(*reinterpret_cast<const v4*>(src), r, (int)off, 0, NT);
+}
+
 } // namespace inter_dev
 } // namespace rr
