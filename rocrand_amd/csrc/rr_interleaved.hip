@@ -33,17 +33,6 @@ __global__ __launch_bounds__(256) void init_kernel(uint32_t* states, const devic
     g.store(states, e);
 }
 
-template<int BYTES>
-__device__ __forceinline__ void store_bytes(void* dst, const void* src)
-{
-    if constexpr(BYTES == 4)
-        *reinterpret_cast<uint32_t*>(dst) = *reinterpret_cast<const uint32_t*>(src);
-    else if constexpr(BYTES == 8)
-        *reinterpret_cast<uint2*>(dst) = *reinterpret_cast<const uint2*>(src);
-    else
-        *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(src);
-}
-
 // Poisson: each block copies the packed alias table into dynamic LDS (size * 8 bytes, no
 // table above lambda 4000) and samples from there.  Every thread of the block calls this
 // before any thread leaves.
@@ -98,14 +87,30 @@ __global__ __launch_bounds__(256) void generate_kernel(interleaved_call c)
                                  || ok == out_kind::uniform_f16)
                                     ? 1
                                     : 4;
+    // The block's last engine takes the fewest vec-steps: that many run for every lane
+    // under a scalar counter, storing from a block-uniform base (buffer stores), and the
+    // rest (at most one) per lane.
+    const uint32_t     id_last = blockIdx.x * blockDim.x + blockDim.x - 1;
+    const uint32_t     ucnt    = id_last < c.vec_n ? (uint32_t)((c.vec_n - id_last + S - 1) / S) : 0u;
+    const uint32_t     off     = id * (uint32_t)BYTES;
+    constexpr uint32_t lim     = S * (uint32_t)BYTES;
+    char*              base    = reinterpret_cast<char*>(vec_data);
 #pragma unroll UNROLL
-    for(; index < c.vec_n; index += S)
+    for(uint32_t k = 0; k < ucnt; k++, index += S, base += (size_t)S * BYTES)
     {
 #pragma unroll
         for(int i = 0; i < IN; i++)
             in[i] = g.next();
         D::apply(in, o, c.params);
-        store_bytes<BYTES>(vec_data + index * OUT, o);
+        store_uniform_base<BYTES>(base, off, lim, o);
+    }
+    for(; index < c.vec_n; index += S, base += (size_t)S * BYTES)
+    {
+#pragma unroll
+        for(int i = 0; i < IN; i++)
+            in[i] = g.next();
+        D::apply(in, o, c.params);
+        store_uniform_base<BYTES>(base, off, lim, o);
     }
     // The engine that would produce vec index vec_n also writes the head and
     // then the tail (mrg.hpp:114-153).
