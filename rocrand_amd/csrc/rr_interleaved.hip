@@ -80,13 +80,14 @@ __global__ __launch_bounds__(256) void generate_kernel(interleaved_call c)
     size_t       index = id;
     // The transforms (and table loads) of consecutive steps overlap when unrolled, which
     // the 2 wavefronts per SIMD of an S-lane grid cannot provide; integer and uniform
-    // outputs are store-bound already.
+    // outputs are store-bound already, except LFSR113's doubles (two of its longer steps
+    // per output: unrolled x2, 4.43 -> 5.31 TB/s; XORWOW's measured 2% slower so).
     constexpr out_kind ok     = dist_kind<D>::value;
     constexpr int      UNROLL = (ok == out_kind::u8 || ok == out_kind::u16 || ok == out_kind::u32
-                                 || ok == out_kind::uniform_f32 || ok == out_kind::uniform_f64
-                                 || ok == out_kind::uniform_f16)
+                                 || ok == out_kind::uniform_f32 || ok == out_kind::uniform_f16)
                                     ? 1
-                                    : 4;
+                                    : ok == out_kind::uniform_f64 ? (K == inter_kind::lfsr113 ? 2 : 1)
+                                                                  : 4;
     // The block's last engine takes the fewest vec-steps: that many run for every lane
     // under a scalar counter, storing from a block-uniform base (buffer stores), and the
     // rest (at most one) per lane.
