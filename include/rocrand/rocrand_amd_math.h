@@ -216,24 +216,31 @@ struct mrg_consts<31>
     static constexpr double uint_norm = MRG31_UINT_NORM;
 };
 
+// (double)v * k, correctly rounded, in one fused multiply-add and no integer-to-double
+// conversion: the double with high word 0x43300000 and low word v is 2^52 + v exactly,
+// and fma(2^52 + v, k, -2^52 k) rounds (2^52 + v) k - 2^52 k = v k once (2^52 k is exact,
+// a power-of-two scaling of k).
+__device__ __forceinline__ double scale_u32(uint32_t v, double k)
+{
+    const double x = __hiloint2double(0x43300000, (int)v);
+    return __builtin_fma(x, k, -0x1p52 * k);
+}
+
 // rocrand_uniform.h:161-219
 template<int MRG>
 __device__ __forceinline__ uint32_t mrg_uint(uint32_t v)
 {
-    ROCRAND_AMD_FP_EXACT
-    return (uint32_t)((double)(v - 1) * mrg_consts<MRG>::uint_norm);
+    return (uint32_t)scale_u32(v - 1, mrg_consts<MRG>::uint_norm);
 }
 template<int MRG>
 __device__ __forceinline__ float mrg_f32(uint32_t v)
 {
-    ROCRAND_AMD_FP_EXACT
-    return (float)((double)v * mrg_consts<MRG>::norm);
+    return (float)scale_u32(v, mrg_consts<MRG>::norm);
 }
 template<int MRG>
 __device__ __forceinline__ double mrg_f64(uint32_t v)
 {
-    ROCRAND_AMD_FP_EXACT
-    return (double)v * mrg_consts<MRG>::norm;
+    return scale_u32(v, mrg_consts<MRG>::norm);
 }
 
 // rocrand_normal.h:51-66 (host branch: separate sinf/cosf of the same angle).
