@@ -90,6 +90,8 @@ int orc_mrg_matrix(int rng_type, int component, int which, int index, unsigned l
 
 /* Poisson alias table for lambda: returns size, writes offset/prob/alias (caller buffers >= 4096). */
 int orc_poisson_table(double lambda, unsigned int* offset, double* prob, unsigned int* alias);
+/* Tables of rocrand_create_discrete_distribution for `size` weights: alias pair and CDF. */
+int orc_discrete_tables(const double* weights, unsigned size, double* prob, unsigned* alias, double* cdf);
 
 /* float -> half, round to nearest even (the host __float2half). */
 unsigned short orc_float2half(float f);

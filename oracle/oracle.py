@@ -68,6 +68,8 @@ def lib():
         L.orc_mrg_matrix.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
         L.orc_poisson_table.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_uint), ctypes.c_void_p,
                                         ctypes.c_void_p]
+        L.orc_discrete_tables.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p]
         L.orc_float2half.argtypes = [ctypes.c_float]
         L.orc_float2half.restype = ctypes.c_ushort
         for name, ct in [("orc_philox4x32_10_block", ctypes.c_uint), ("orc_threefry4x32_20_block", ctypes.c_uint),
@@ -130,6 +132,11 @@ def ref():
         R.ref_scramble64_constants.restype = ctypes.POINTER(ctypes.c_ulonglong)
         R.ref_mt19937_jump.restype = ctypes.POINTER(ctypes.c_uint)
         R.ref_mt19937_set_limit.argtypes = [ctypes.c_void_p, ctypes.c_uint]
+        R.ref_discrete_tables.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p]
+        R.ref_poisson_tables.restype = ctypes.c_uint
+        R.ref_poisson_tables.argtypes = [ctypes.c_double, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint),
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         # the reference tree has no 64-bit direction-vector tables: hand it the oracle's
         t = sobol_tables()
         _ref_keep.extend([t[2], t[3]])
@@ -229,3 +236,36 @@ def poisson_table(lam):
     alias = np.zeros(8192, dtype=np.uint32)
     size = lib().orc_poisson_table(lam, ctypes.byref(off), prob.ctypes.data, alias.ctypes.data)
     return off.value, prob[:size].copy(), alias[:size].copy()
+
+
+def discrete_tables(weights):
+    """(prob, alias, cdf) of rocrand_create_discrete_distribution for `weights`."""
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    prob, cdf = np.zeros(w.size), np.zeros(w.size)
+    alias = np.zeros(w.size, dtype=np.uint32)
+    if lib().orc_discrete_tables(w.ctypes.data, w.size, prob.ctypes.data, alias.ctypes.data, cdf.ctypes.data):
+        raise MemoryError("orc_discrete_tables")
+    return prob, alias, cdf
+
+
+def ref_discrete_tables(weights, offset=0):
+    """The reference factory's (prob, alias, cdf) for `weights` (oracle/_ref/libref.so)."""
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    prob, cdf = np.zeros(w.size), np.zeros(w.size)
+    alias = np.zeros(w.size, dtype=np.uint32)
+    if ref().ref_discrete_tables(w.ctypes.data, w.size, offset, prob.ctypes.data, alias.ctypes.data,
+                                 cdf.ctypes.data):
+        raise RuntimeError("reference discrete factory failed")
+    return prob, alias, cdf
+
+
+def ref_poisson_tables(lam, cap=1 << 16):
+    """The reference's (offset, prob, alias, cdf) for a Poisson distribution of mean `lam`."""
+    off = ctypes.c_uint(0)
+    prob, cdf = np.zeros(cap), np.zeros(cap)
+    alias = np.zeros(cap, dtype=np.uint32)
+    size = ref().ref_poisson_tables(lam, cap, ctypes.byref(off), prob.ctypes.data, alias.ctypes.data,
+                                    cdf.ctypes.data)
+    if size == 0 or size > cap:
+        raise RuntimeError(f"reference Poisson table for {lam}: size {size}")
+    return off.value, prob[:size].copy(), alias[:size].copy(), cdf[:size].copy()

@@ -238,15 +238,19 @@ static void poisson_probs(double lambda, double** pout, unsigned* size, unsigned
     *pout   = p;
 }
 
-static void build_alias(double* p, unsigned size, double* prob, unsigned* alias)
+/* normalize, discrete.hpp:189-200 */
+static void normalize(double* p, unsigned size)
 {
-    /* normalize, discrete.hpp:189-200 */
     double sum = 0.0;
     for(unsigned i = 0; i < size; i++)
         sum += p[i];
     for(unsigned i = 0; i < size; i++)
         p[i] /= sum;
-    /* Vose alias, LIFO stacks, discrete.hpp:212-265 */
+}
+
+/* Vose alias over normalized p (consumed), LIFO stacks, discrete.hpp:212-265 */
+static void vose_alias(double* p, unsigned size, double* prob, unsigned* alias)
+{
     const double average = 1.0 / size;
     unsigned*    small   = (unsigned*)malloc(size * sizeof(unsigned));
     unsigned*    large   = (unsigned*)malloc(size * sizeof(unsigned));
@@ -278,6 +282,32 @@ static void build_alias(double* p, unsigned size, double* prob, unsigned* alias)
         prob[large[i]] = 1.0;
     free(small);
     free(large);
+}
+
+static void build_alias(double* p, unsigned size, double* prob, unsigned* alias)
+{
+    normalize(p, size);
+    vose_alias(p, size, prob, alias);
+}
+
+int orc_discrete_tables(const double* weights, unsigned size, double* prob, unsigned* alias, double* cdf)
+{
+    /* discrete.hpp:111-143 create(): normalize once, then the alias table from a copy and
+     * the running sums (create_cdf, discrete.hpp:277-286) from the normalized weights */
+    double* p = (double*)malloc(size * sizeof(double));
+    if(p == NULL)
+        return -1;
+    memcpy(p, weights, size * sizeof(double));
+    normalize(p, size);
+    double sum = 0.0;
+    for(unsigned i = 0; i < size; i++)
+    {
+        sum += p[i];
+        cdf[i] = sum;
+    }
+    vose_alias(p, size, prob, alias);
+    free(p);
+    return 0;
 }
 
 int orc_poisson_table(double lambda, unsigned int* offset, double* prob, unsigned int* alias)

@@ -822,4 +822,37 @@ int ref_engine_stream(int type, unsigned long long seed, unsigned long long sub,
     return -1;
 }
 
+// The tables rocrand_create_discrete_distribution builds for `size` user weights
+// (rocrand.cpp:506-533): the reference's own factory, host-side instance
+// (discrete.hpp:111-143 create: normalize, create_alias_table, create_cdf).
+// Returns 0, or -1 when the factory reports an error.
+int ref_discrete_tables(const double* weights, unsigned int size, unsigned int offset,
+                        double* prob, unsigned int* alias, double* cdf)
+{
+    using F = discrete_distribution_factory<DISCRETE_METHOD_UNIVERSAL, true>;
+    rocrand_discrete_distribution_st st{};
+    if(F::create(std::vector<double>(weights, weights + size), size, offset, st)
+       != ROCRAND_STATUS_SUCCESS)
+        return -1;
+    std::copy(st.probability, st.probability + size, prob);
+    std::copy(st.alias, st.alias + size, alias);
+    std::copy(st.cdf, st.cdf + size, cdf);
+    F::deallocate(st);
+    return 0;
+}
+
+// The tables rocrand_create_poisson_distribution builds for `lambda` (rocrand.cpp:462-500:
+// calculate_poisson_probabilities, poisson.hpp:110-161, then the same factory).  Returns the
+// table size and writes its offset; writes the tables only when the size is at most `cap`.
+unsigned int ref_poisson_tables(double lambda, unsigned int cap, unsigned int* offset,
+                                double* prob, unsigned int* alias, double* cdf)
+{
+    unsigned int              size, off;
+    const std::vector<double> p = calculate_poisson_probabilities(lambda, size, off);
+    *offset                     = off;
+    if(size > cap)
+        return size;
+    return ref_discrete_tables(p.data(), size, off, prob, alias, cdf) == 0 ? size : 0;
+}
+
 } // extern "C"

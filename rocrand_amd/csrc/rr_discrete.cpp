@@ -22,23 +22,26 @@ rocrand_status make_device_distribution(const std::vector<double>& p, unsigned i
     rocrand_discrete_distribution_st h{};
     h.size   = size;
     h.offset = offset;
-    if(hipMalloc(&h.probability, size * sizeof(double)) != hipSuccess
-       || hipMalloc(&h.alias, size * sizeof(unsigned int)) != hipSuccess
-       || hipMalloc(&h.cdf, size * sizeof(double)) != hipSuccess)
-    {
+    rocrand_discrete_distribution handle = nullptr;
+    // Every failure path releases whatever was allocated before it.
+    auto fail = [&](rocrand_status status) {
         (void)hipFree(h.probability);
         (void)hipFree(h.alias);
         (void)hipFree(h.cdf);
-        return ROCRAND_STATUS_ALLOCATION_FAILED;
-    }
+        (void)hipFree(handle);
+        return status;
+    };
+    if(hipMalloc(&h.probability, size * sizeof(double)) != hipSuccess
+       || hipMalloc(&h.alias, size * sizeof(unsigned int)) != hipSuccess
+       || hipMalloc(&h.cdf, size * sizeof(double)) != hipSuccess
+       || hipMalloc(&handle, sizeof(rocrand_discrete_distribution_st)) != hipSuccess)
+        return fail(ROCRAND_STATUS_ALLOCATION_FAILED);
     if(hipMemcpy(h.probability, prob.data(), size * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
        || hipMemcpy(h.alias, alias.data(), size * sizeof(unsigned int), hipMemcpyHostToDevice) != hipSuccess
-       || hipMemcpy(h.cdf, cdf.data(), size * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-        return ROCRAND_STATUS_INTERNAL_ERROR;
-    if(hipMalloc(out, sizeof(rocrand_discrete_distribution_st)) != hipSuccess)
-        return ROCRAND_STATUS_ALLOCATION_FAILED;
-    if(hipMemcpy(*out, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess)
-        return ROCRAND_STATUS_INTERNAL_ERROR;
+       || hipMemcpy(h.cdf, cdf.data(), size * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
+       || hipMemcpy(handle, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(ROCRAND_STATUS_INTERNAL_ERROR);
+    *out = handle;
     return ROCRAND_STATUS_SUCCESS;
 }
 

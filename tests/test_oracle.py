@@ -50,6 +50,73 @@ def test_threefry2_known_answers():
                                                                                 0x6F81ED42F350084D]
 
 
+_TF4_ROT = {32: [(10, 26), (11, 21), (13, 27), (23, 5), (6, 20), (17, 11), (25, 10), (18, 20)],
+            64: [(14, 16), (52, 57), (23, 40), (5, 37), (25, 33), (46, 12), (58, 22), (32, 32)]}
+_TF_PARITY = {32: 0x1BD11BDA, 64: 0x1BD11BDAA9FC1A22}
+
+
+def _threefry4_block(ctr, key, w, pair_switch_bit):
+    """Threefry-4xW with 20 rounds, Skein key schedule (injection every 4 rounds).
+
+    Word pairs (0,1),(2,3) mix in a round whose `pair_switch_bit` bit is clear and (0,3),(2,1)
+    otherwise.  Random123 alternates every round (bit 1); the reference switches every two
+    rounds (`(round_idx & 2u) == 0`, rocrand_threefry4_impl.h:210-229).
+    """
+    m = (1 << w) - 1
+    ks = list(key) + [_TF_PARITY[w] ^ key[0] ^ key[1] ^ key[2] ^ key[3]]
+    x = [(c + k) & m for c, k in zip(ctr, key)]
+
+    def rotl(v, r):
+        return ((v << r) | (v >> (w - r))) & m
+
+    for r in range(20):
+        a, b = _TF4_ROT[w][r & 7]
+        if not r & pair_switch_bit:
+            x[0] = (x[0] + x[1]) & m
+            x[1] = rotl(x[1], a) ^ x[0]
+            x[2] = (x[2] + x[3]) & m
+            x[3] = rotl(x[3], b) ^ x[2]
+        else:
+            x[0] = (x[0] + x[3]) & m
+            x[3] = rotl(x[3], a) ^ x[0]
+            x[2] = (x[2] + x[1]) & m
+            x[1] = rotl(x[1], b) ^ x[2]
+        if r & 3 == 3:
+            i = r // 4
+            x = [(x[j] + ks[(1 + i + j) % 5]) & m for j in range(4)]
+            x[3] = (x[3] + 1 + i) & m
+    return x
+
+
+PI64 = [0x243F6A8885A308D3, 0x13198A2E03707344, 0xA4093822299F31D0, 0x082EFA98EC4E6C89,
+        0x452821E638D01377, 0xBE5466CF34E90C6C, 0xC0AC29B7C97C50DD, 0x3F84D5B5B5470917]
+# Random123 kat_vectors, threefry4x32 / threefry4x64 with 20 rounds.
+_TF4_KATS = [
+    (32, [0] * 4, [0] * 4, [0x9C6CA96A, 0xE17EAE66, 0xFC10ECD4, 0x5256A7D8]),
+    (32, [0xFFFFFFFF] * 4, [0xFFFFFFFF] * 4, [0x2A881696, 0x57012287, 0xF6C7446E, 0xA16A6732]),
+    (32, PI32[:4], PI32[4:6] + [0x082EFA98, 0xEC4E6C89], [0x59CD1DBB, 0xB8879579, 0x86B5D00C, 0xAC8B6D84]),
+    (64, [0] * 4, [0] * 4, [0x09218EBDE6C85537, 0x55941F5266D86105, 0x4BD25E16282434DC, 0xEE29EC846BD2E40B]),
+    (64, [2**64 - 1] * 4, [2**64 - 1] * 4,
+     [0x29C24097942BBA1B, 0x0371BBFB0F6F4E11, 0x3C231FFA33F83A1C, 0xCD29113FDE32D168]),
+]
+
+
+@pytest.mark.parametrize("w,ctr,key,kat", _TF4_KATS)
+def test_threefry4_rounds_pinned(w, ctr, key, kat):
+    # The restatement above reproduces Random123's published vectors with Random123's
+    # pairing, which fixes the rotation constants, the parity word and the key injection;
+    # with the reference's pairing it must then equal the oracle's block function.  A
+    # wrong rotation, injection index or pairing fails one of the two.
+    assert _threefry4_block(ctr, key, w, 1) == kat
+    fn = "orc_threefry4x32_20_block" if w == 32 else "orc_threefry4x64_20_block"
+    assert _block(fn, ctr, key, w64=(w == 64)) == _threefry4_block(ctr, key, w, 2)
+    rng = np.random.default_rng(w)
+    for _ in range(8):
+        c = [int(v) for v in rng.integers(0, 2**w, 4, dtype=np.uint64 if w == 64 else np.uint32)]
+        k = [int(v) for v in rng.integers(0, 2**w, 4, dtype=np.uint64 if w == 64 else np.uint32)]
+        assert _block(fn, c, k, w64=(w == 64)) == _threefry4_block(c, k, w, 2)
+
+
 def test_philox_engine_stream_is_counter_blocks():
     # word k of engine(seed) = block(counter = k/4, key = (seed lo, seed hi))[k % 4]
     seed = 0x0123456789ABCDEF
@@ -212,6 +279,57 @@ def test_oracle_equals_reference_kernel_bodies(rng, kind):
             s2, x2 = g2.generate(kind, n, addr, a, b)
             assert s1 == s2 == 0
             assert x1.tobytes() == x2.tobytes()
+
+
+@ref_only
+@pytest.mark.parametrize("lam", [0.01, 0.5, 1.0, 4.0, 10.0, 63.5, 64.0, 100.0, 999.9, 1000.0, 2500.0, 3999.5, 4000.0])
+def test_poisson_alias_layout_equals_reference(lam):
+    # The alias table's layout (which column holds which alias, the LIFO order of the
+    # small/large stacks) is not fixed by the pmf; the reference's own construction
+    # (poisson.hpp:110-161, discrete.hpp:189-286) fixes it.
+    off, prob, alias = O.poisson_table(lam)
+    roff, rprob, ralias, rcdf = O.ref_poisson_tables(lam)
+    assert off == roff
+    assert prob.tobytes() == rprob.tobytes()
+    assert np.array_equal(alias, ralias)
+    assert rcdf[-1] == pytest.approx(1.0, abs=1e-12)
+
+
+def _weights(kind, size, rng):
+    if kind == "uniform":
+        return rng.random(size)
+    if kind == "skewed":
+        return rng.random(size) ** 8
+    if kind == "sparse":
+        w = rng.random(size)
+        w[rng.random(size) < 0.5] = 0.0
+        w[0] = max(w[0], 1e-3)
+        return w
+    return np.full(size, 3.0)  # all equal
+
+
+@ref_only
+@pytest.mark.parametrize("size", [1, 2, 7, 100, 1000, 4096])
+@pytest.mark.parametrize("kind", ["uniform", "skewed", "sparse", "equal"])
+def test_discrete_tables_equal_reference(size, kind):
+    w = _weights(kind, size, np.random.default_rng(size))
+    prob, alias, cdf = O.discrete_tables(w)
+    rprob, ralias, rcdf = O.ref_discrete_tables(w, 5)
+    assert prob.tobytes() == rprob.tobytes()
+    assert np.array_equal(alias, ralias)
+    assert cdf.tobytes() == rcdf.tobytes()
+
+
+@pytest.mark.parametrize("size", [1, 3, 100, 4096])
+def test_discrete_tables_recover_weights(size):
+    # closed form: the alias pair puts mass prob[j] on column j and 1 - prob[j] on alias[j]
+    w = _weights("skewed", size, np.random.default_rng(7 + size))
+    prob, alias, cdf = O.discrete_tables(w)
+    mass = prob.copy()
+    np.add.at(mass, alias, 1.0 - prob)
+    assert np.max(np.abs(mass / size - w / w.sum())) < 1e-12
+    assert np.max(np.abs(cdf - np.cumsum(w / w.sum()))) < 1e-12
+    assert np.all((prob >= 0) & (prob <= 1.0)) and np.all(alias < size)
 
 
 @ref_only
