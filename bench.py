@@ -6,12 +6,15 @@
 
 A step is one rocrand_generate_uniform call (Philox4x32-10, seed 12345,
 offset 0) writing N = 2^28 fp32 samples into a preallocated HBM buffer: the
-plain bulk path per engine at 2^28 samples.  The JSON line's `value` is that
-call's output bandwidth in decimal GB/s; `per_generator` times the other
-engine x distribution pairs of the same workload; `roofline` prices the
-kernel against MI355X's 8 TB/s HBM peak (and a measured store-kernel ceiling);
-`cpu_baseline` times the reference's own kernel bodies on the host CPU over
-the same workload.
+plain bulk path per engine at 2^28 samples.  Before the W warm-up steps, the step
+runs for 0.5 s of wall time (`settle_calls`) so that the GPU's clocks have left their
+idle level.  The JSON line's `value` is that call's output bandwidth in decimal GB/s;
+`per_generator` times the other engine x distribution pairs of the same workload;
+`roofline` prices the kernel against MI355X's 8 TB/s HBM peak (and a measured
+store-kernel ceiling); `normals_accuracy` gives the normal fp32 outputs' error against
+a float64 Box-Muller of the same words; `cpu_baseline` times the reference's
+host-generator path (the installed rocRAND's, on every available core, and the
+reference's own kernel bodies on one core) over the same workload.
 """
 from __future__ import annotations
 
@@ -46,6 +49,22 @@ def time_calls(fn, stream, steps, warmup):
     end.record(stream)
     torch.cuda.synchronize()
     return start.elapsed_time(end) / steps
+
+
+def settle(fn, seconds=0.5, chunk=50):
+    """Run `fn` in chunks until `seconds` of wall time have passed; returns the call count.
+    A GPU that was idle runs its first calls at low clocks while the power management
+    raises them (the first calls of a fresh process took up to 1.6x the steady-state time);
+    this phase brings it to the steady state before the warm-up and timed steps."""
+    import torch
+
+    calls, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(chunk):
+            fn()
+        torch.cuda.synchronize()
+        calls += chunk
+    return calls
 
 
 def _pairs(R, torch):
@@ -214,6 +233,54 @@ def serving_bench(torch, rates=(1000, 10000, 50000), seconds=1.0, budget_gb=32):
     return {"budget_GB": budget_gb, "workload_seed": 11, "runs": out}
 
 
+def _exact_box_muller_f32(u, v):
+    import numpy as np
+
+    s = np.sqrt(-2.0 * np.log(u.astype(np.float64)))
+    out = np.empty((u.size, 2))
+    out[:, 0] = np.sin(v.astype(np.float64)) * s
+    out[:, 1] = np.cos(v.astype(np.float64)) * s
+    return out.ravel()
+
+
+def normals_accuracy(R, torch, n=1 << 22, seed=12345):
+    """Max and mean error, in ulps, of the device's normal fp32 outputs (mean 0, stddev 1)
+    against the Box-Muller of the same engine words evaluated in float64: Philox4x32-10
+    (words from the u32 output, consecutive pairs) and MRG32k3a (uniforms from the uniform
+    fp32 output; normal vec v comes from engine v mod 131072, words 2q, 2q + 1, q = v div
+    131072)."""
+    import numpy as np
+
+    res = {}
+    for name, rng in (("philox4x32_10", R.PHILOX4_32_10), ("mrg32k3a", R.MRG32K3A)):
+        g = R.PRNG(rng, seed=seed)
+        z = torch.empty(n, dtype=torch.float32, device="cuda")
+        g.normal(z, 0.0, 1.0)
+        if rng == R.PHILOX4_32_10:
+            w = torch.empty(n, dtype=torch.uint32, device="cuda")
+            R.PRNG(rng, seed=seed).generate(w)
+            w = w.cpu().numpy().reshape(-1, 2)
+            c, c2pi = np.float32(2.3283064e-10), np.float32(1.46291807e-09)
+            u = c + w[:, 0].astype(np.float32) * c
+            v = c2pi + w[:, 1].astype(np.float32) * c2pi
+        else:
+            f = torch.empty(n, dtype=torch.float32, device="cuda")
+            R.PRNG(rng, seed=seed).uniform(f)
+            f = f.cpu().numpy()
+            S = 131072
+            vec = np.arange(n // 2)
+            e, q = vec % S, vec // S
+            u = f[e + 2 * q * S]
+            v = f[e + (2 * q + 1) * S] * np.float32(6.2831855)
+        torch.cuda.synchronize()
+        exact = _exact_box_muller_f32(u, v)
+        got = z.cpu().numpy().astype(np.float64)
+        ulp = np.spacing(np.abs(exact.astype(np.float32))).astype(np.float64)
+        err = np.abs(got - exact) / ulp
+        res[name] = {"max_ulp": round(float(err.max()), 4), "mean_ulp": round(float(err.mean()), 4)}
+    return {"against": "float64 Box-Muller of the same words", "n": n, "seed": seed, "normal_f32": res}
+
+
 def load_traffic():
     """HBM bytes per launch of the headline kernel, from the committed counter summary."""
     path = os.path.join(HERE, "profiles", "traffic_philox_uniform_f32_2e28.json")
@@ -224,32 +291,82 @@ def load_traffic():
         return None
 
 
+def _installed_rocrand_host(n):
+    """The image's installed rocRAND (its host-generator path, rocrand_create_generator_host_blocking):
+    Philox4x32-10, seed 12345, n uniform fp32 into host memory; median of 3 timed calls after
+    one warm-up.  Returns (seconds, version) or None when the library is absent."""
+    import ctypes
+
+    import numpy as np
+
+    path = "/opt/rocm/lib/librocrand.so"
+    if not os.path.exists(path):
+        return None
+    try:
+        L = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+    except OSError:
+        return None
+    gen = ctypes.c_void_p()
+    if L.rocrand_create_generator_host_blocking(ctypes.byref(gen), 404) != 0:
+        return None
+    ver = ctypes.c_int(0)
+    L.rocrand_get_version(ctypes.byref(ver))
+    L.rocrand_set_seed(gen, ctypes.c_ulonglong(12345))
+    out = np.empty(n, dtype=np.float32)
+    ptr = ctypes.c_void_p(out.ctypes.data)
+    L.rocrand_generate_uniform(gen, ptr, ctypes.c_size_t(1 << 20))
+    times = []
+    for _ in range(3):
+        t = time.perf_counter()
+        st = L.rocrand_generate_uniform(gen, ptr, ctypes.c_size_t(n))
+        times.append(time.perf_counter() - t)
+        if st != 0:
+            L.rocrand_destroy_generator(gen)
+            return None
+    L.rocrand_destroy_generator(gen)
+    return sorted(times)[1], ver.value
+
+
 def cpu_baseline(n):
-    """The reference's Philox kernel bodies on one host core (oracle/_ref), else the oracle port."""
+    """The reference's host-generator CPU path on this box's host cores.
+
+    Primary: the image's installed rocRAND host generator (same path, a later release), on
+    every core this process may use.  Also: the reference's own Philox kernel bodies
+    (compiled here from its sources, oracle/_ref) on one core."""
     from oracle import oracle as O
 
-    kind = "reference"
+    cores = len(os.sched_getaffinity(0))
+    installed = _installed_rocrand_host(n)
+    g, kind_ref = None, "reference"
     try:
         g = O.RefGen(O.PHILOX4_32_10, seed=12345)
     except Exception:
-        g = O.OracleGen(O.PHILOX4_32_10, seed=12345)
-        kind = "port"
+        g, kind_ref = O.OracleGen(O.PHILOX4_32_10, seed=12345), "port"
     g.generate("uniform_f32", 1024, 0x1000)
     t = time.perf_counter()
     st, _ = g.generate("uniform_f32", n, 0x1000)
-    dt = time.perf_counter() - t
+    dt1 = time.perf_counter() - t
     assert st == 0
-    return {"value": round(n * 4 / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": kind,
-            "sample": f"Philox4x32-10 uniform fp32, seed 12345, n=2^{n.bit_length() - 1} on one host core "
-                      f"({n / dt / 1e6:.1f} Msamples/s, {dt:.1f} s)"}
+    one_core = {"value": round(n * 4 / dt1 / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": kind_ref,
+                "sample": f"reference kernel bodies compiled from its sources (oracle/_ref), Philox4x32-10 "
+                          f"uniform fp32, seed 12345, n=2^{n.bit_length() - 1}, {dt1:.2f} s"}
+    if installed is None:
+        one_core["secondary"] = None
+        return one_core
+    dt, ver = installed
+    return {"value": round(n * 4 / dt / 1e9, 4), "unit": "GB/s", "cores": cores, "kind": "reference",
+            "sample": f"installed rocRAND {ver // 100000}.{ver // 100 % 1000}.{ver % 100} host generator "
+                      f"(rocrand_create_generator_host_blocking), Philox4x32-10 uniform fp32, seed 12345, "
+                      f"n=2^{n.bit_length() - 1} into host memory, median of 3 calls ({dt:.2f} s), "
+                      f"{cores} cores available ({os.cpu_count()} online)",
+            "one_core_reference_bodies": one_core}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    # ~0.5 s of untimed calls: the first calls after process start run at lower clocks
-    ap.add_argument("--warmup", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=1 << 28)
     ap.add_argument("--no-table", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -285,6 +402,7 @@ def main():
     def step():
         g.uniform(out)
 
+    settle_calls = settle(step)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -303,6 +421,7 @@ def main():
 
     ceiling = store_ceiling(torch, stream, n, 10)
     table = None if args.no_table else per_generator_table(R, torch, stream, n, 10)
+    accuracy = normals_accuracy(R, torch)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
     batched = None if args.no_batched else batched_bench(R, torch)
     if batched:
@@ -316,6 +435,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "settle_calls": settle_calls,
         "ms_per_step": round(ms, 4),
         "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
         "higher_is_better": True,
@@ -329,6 +449,7 @@ def main():
                      "frac": round(value / HBM_PEAK_GBS, 4), "traffic": load_traffic(),
                      "measured_store_ceiling_gbs": round(ceiling, 1)},
         "cpu_baseline": cpu,
+        "normals_accuracy": accuracy,
         "per_generator": table,
         "batched": batched,
         "serving": serving,

@@ -243,17 +243,80 @@ __device__ __forceinline__ double mrg_f64(uint32_t v)
     return scale_u32(v, mrg_consts<MRG>::norm);
 }
 
-// rocrand_normal.h:51-66 (host branch: separate sinf/cosf of the same angle).
+// The Box-Muller pair sin(v) sqrt(-2 ln u), cos(v) sqrt(-2 ln u) for a float u in
+// [2^-32, 1] and a float angle v in (0, 8), evaluated as one expression in double and
+// rounded to float once.  Relative error before that rounding is below 2^-36 (log1p series
+// to r^4 on |r| < 2^-7: 2^-30 relative on ln u, halved by the square root; sin series to
+// r^5 and cos to r^4 on |r| <= pi/64; a float reciprocal square root refined by one Newton
+// step), so each output is within 0.51 ulp of the exact value.  The reference's float
+// pipeline (logf, sqrtf, sinf/cosf, product: four roundings, rocrand_normal.h:51-66) lies
+// within 2.4 ulp of the exact value, so the two stay within 2 ulp of each other.
+// u = 1 gives ln u = +0 (table entry {1, 0}), x = -0 and the radius -0, as the reference's
+// sqrtf(-2.0f * logf(1.0f)); the clamp keeps the reciprocal square root finite there.
+__device__ __forceinline__ void box_muller_uv(float u, float v, float& r0, float& r1)
+{
+    ROCRAND_AMD_FP_EXACT
+    const uint32_t ix  = __float_as_uint(u);
+    const uint32_t tmp = ix - RR_LOGTAB_OFF;
+    const uint32_t i   = (tmp >> (23 - RR_LOGTAB_BITS)) & ((1u << RR_LOGTAB_BITS) - 1u);
+    const int      k   = (int)tmp >> 23;
+    const float    z   = __uint_as_float(ix - (tmp & 0xff800000u));
+    const double2  lt  = rr_logtab[i];
+    const double   r   = __builtin_fma((double)z, lt.x, -1.0);
+    const double   r2  = r * r;
+    double         q   = __builtin_fma(r, -0.25, 1.0 / 3);
+    q                  = __builtin_fma(q, r, -0.5);
+    const double   lnu = __builtin_fma((double)k, RR_LN2, lt.y) + __builtin_fma(q, r2, r);
+    const double   x   = -2.0 * lnu;
+    const double   y   = (double)__builtin_amdgcn_rsqf(__builtin_fmaxf((float)x, 0x1p-126f));
+    const double   t   = x * y;
+    const double   h   = __builtin_fma(-t, y, 1.0);
+    const double   s   = __builtin_fma(t * h, 0.5, t);
+
+    const double  vd = (double)v;
+    const double  qv = __builtin_rint(vd * RR_SC_INV_STEP);
+    const int     kv = (int)qv & ((1 << RR_SCTAB_BITS) - 1);
+    double        a  = __builtin_fma(-qv, RR_SC_STEP_HI, vd);
+    a                = __builtin_fma(-qv, RR_SC_STEP_LO, a);
+    const double2 st = rr_sctab[kv];
+    const double  a2 = a * a;
+    const double  sa = __builtin_fma(a * a2, __builtin_fma(a2, 1.0 / 120, -1.0 / 6), a);
+    const double  ca = __builtin_fma(a2, __builtin_fma(a2, 1.0 / 24, -0.5), 1.0);
+    const double  sn = __builtin_fma(st.x, ca, st.y * sa);
+    const double  cs = __builtin_fma(st.y, ca, -(st.x * sa));
+    r0               = (float)(sn * s);
+    r1               = (float)(cs * s);
+}
+
+// rocrand_normal.h:51-66: u = 2^-32 + x 2^-32, angle 2^-32 2pi + y 2^-32 2pi (rounded as
+// the reference rounds them), then the pair above.
 __device__ __forceinline__ void box_muller(uint32_t x, uint32_t y, float& r0, float& r1)
 {
     ROCRAND_AMD_FP_EXACT
     const float u = uniform_f32(x);
     const float v = R_2POW32_INV_2PI + (float)y * R_2POW32_INV_2PI;
+    box_muller_uv(u, v, r0, r1);
+}
+
+// The same pair through the reference's own steps -- logf, sqrtf, sinf/cosf, each rounded
+// to float (correctly) -- for the log-normal transforms: exp magnifies a one-ulp change
+// of the variate to several ulps of the output, and these steps reproduce the reference
+// host path's variate bit for bit wherever its libm rounds correctly.
+__device__ __forceinline__ void box_muller_uv_stepwise(float u, float v, float& r0, float& r1)
+{
+    ROCRAND_AMD_FP_EXACT
     const float s = sqrtf_radius(-2.0f * logf_tab(u));
     float       sn, cs;
     sincosf_acc(v, &sn, &cs);
     r0 = sn * s;
     r1 = cs * s;
+}
+__device__ __forceinline__ void box_muller_stepwise(uint32_t x, uint32_t y, float& r0, float& r1)
+{
+    ROCRAND_AMD_FP_EXACT
+    const float u = uniform_f32(x);
+    const float v = R_2POW32_INV_2PI + (float)y * R_2POW32_INV_2PI;
+    box_muller_uv_stepwise(u, v, r0, r1);
 }
 // rocrand_normal.h:76-96 (host branch: sin(w * pi), not sincospi).
 __device__ __forceinline__ void box_muller_double(uint32_t a, uint32_t b, uint32_t c, uint32_t d, double& r0,
@@ -288,13 +351,13 @@ template<int MRG>
 __device__ __forceinline__ void mrg_box_muller(uint32_t x, uint32_t y, float& r0, float& r1)
 {
     ROCRAND_AMD_FP_EXACT
-    const float u = mrg_f32<MRG>(x);
-    const float v = mrg_f32<MRG>(y) * R_2PI;
-    const float s = sqrtf_radius(-2.0f * logf_tab(u));
-    float       sn, cs;
-    sincosf_acc(v, &sn, &cs);
-    r0 = sn * s;
-    r1 = cs * s;
+    box_muller_uv(mrg_f32<MRG>(x), mrg_f32<MRG>(y) * R_2PI, r0, r1);
+}
+template<int MRG>
+__device__ __forceinline__ void mrg_box_muller_stepwise(uint32_t x, uint32_t y, float& r0, float& r1)
+{
+    ROCRAND_AMD_FP_EXACT
+    box_muller_uv_stepwise(mrg_f32<MRG>(x), mrg_f32<MRG>(y) * R_2PI, r0, r1);
 }
 // rocrand_normal.h:156-172: float-precision uniforms widened to double.
 template<int MRG>

@@ -10,8 +10,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <new>
-#include <string.h>
 #include <vector>
 
 using namespace rr;
@@ -252,8 +252,12 @@ rocrand_status poisson_cache::get(double lambda, hipStream_t stream, dist_params
 // ============================================================================
 rocrand_generator_base_type::~rocrand_generator_base_type()
 {
+    if(m_staged)
+        (void)hipStreamSynchronize(m_staged_stream); // pending host-function copies read m_pinned
     if(m_staging)
         (void)hipFree(m_staging);
+    if(m_pinned)
+        (void)hipHostFree(m_pinned);
 }
 
 rocrand_status rocrand_generator_base_type::set_seed_uint4(uint4)
@@ -311,42 +315,104 @@ rocrand_status rocrand_generator_base_type::make_params(out_kind k, const gen_ar
     return ROCRAND_STATUS_SUCCESS;
 }
 
+// Device staging of `bytes` whose address has the host pointer's alignment modulo 128
+// bytes (the reference's head/tail split depends on the output address), plus pinned
+// staging of the same size for stream-ordered host generators.
+rocrand_status rocrand_generator_base_type::stage_host(void* data, size_t bytes, void*& dev)
+{
+    const size_t need        = bytes + 256;
+    const bool   grow_dev    = need > m_staging_bytes;
+    const bool   grow_pinned = host_mode == 1 && bytes > m_pinned_bytes;
+    if(m_staged && (grow_dev || grow_pinned || m_staged_stream != stream))
+    {
+        // earlier calls' kernels, copies and host functions still use the buffers
+        if(hipStreamSynchronize(m_staged_stream) != hipSuccess)
+            return ROCRAND_STATUS_INTERNAL_ERROR;
+        m_staged = false;
+    }
+    if(grow_dev)
+    {
+        (void)hipFree(m_staging);
+        m_staging       = nullptr;
+        m_staging_bytes = 0;
+        if(hipMalloc(&m_staging, need) != hipSuccess)
+            return ROCRAND_STATUS_ALLOCATION_FAILED;
+        m_staging_bytes = need;
+    }
+    if(grow_pinned)
+    {
+        (void)hipHostFree(m_pinned);
+        m_pinned       = nullptr;
+        m_pinned_bytes = 0;
+        if(hipHostMalloc(&m_pinned, bytes, hipHostMallocDefault) != hipSuccess)
+            return ROCRAND_STATUS_ALLOCATION_FAILED;
+        m_pinned_bytes = bytes;
+    }
+    const uintptr_t base = (reinterpret_cast<uintptr_t>(m_staging) + 127) & ~uintptr_t(127);
+    dev                  = reinterpret_cast<void*>(base + (reinterpret_cast<uintptr_t>(data) & 127));
+    return ROCRAND_STATUS_SUCCESS;
+}
+
+namespace
+{
+// The host-function step of a stream-ordered host generator: the kernel's output has
+// reached pinned memory when the stream runs it (library/src/rng/system.hpp:195-207 runs
+// the kernel bodies themselves from such a callback).
+struct host_copy
+{
+    void*       dst;
+    const void* src;
+    size_t      bytes;
+    static void run(void* p)
+    {
+        const host_copy* c = static_cast<const host_copy*>(p);
+        std::memcpy(c->dst, c->src, c->bytes);
+        delete c;
+    }
+};
+} // namespace
+
 rocrand_status rocrand_generator_base_type::generate(out_kind k, void* data, size_t n, const gen_args& args)
 {
     if(host_mode == 0 || data == nullptr)
         return generate_device(k, data, n, args);
-    // Host generator: run the device path into staging memory whose address has the
-    // host pointer's alignment (the head/tail split depends on it), then copy out.
+    // Host generator: run the device path into staging memory, then copy out.  Blocking
+    // generators (rocrand_create_generator_host_blocking) return with the values written;
+    // the others (rocrand_create_generator_host) return at once and the values are written
+    // when the generator's stream reaches the call, as the reference's host-function launch.
     dist_shape     s;
     rocrand_status st = shape(k, s);
     if(st != ROCRAND_STATUS_SUCCESS)
         return st;
     const size_t bytes = n * s.esz;
-    const size_t need  = bytes + 256;
-    if(need > m_staging_bytes)
-    {
-        if(m_staging)
-        {
-            (void)hipStreamSynchronize(stream);
-            (void)hipFree(m_staging);
-            m_staging = nullptr;
-        }
-        if(hipMalloc(&m_staging, need) != hipSuccess)
-        {
-            m_staging_bytes = 0;
-            return ROCRAND_STATUS_ALLOCATION_FAILED;
-        }
-        m_staging_bytes = need;
-    }
-    const uintptr_t base = (reinterpret_cast<uintptr_t>(m_staging) + 127) & ~uintptr_t(127);
-    void*           dev  = reinterpret_cast<void*>(base + (reinterpret_cast<uintptr_t>(data) & 127));
-    st                   = generate_device(k, dev, n, args);
+    void*        dev   = nullptr;
+    st                 = stage_host(data, bytes, dev);
     if(st != ROCRAND_STATUS_SUCCESS)
         return st;
-    if(bytes > 0 && hipMemcpyAsync(data, dev, bytes, hipMemcpyDeviceToHost, stream) != hipSuccess)
+    st = generate_device(k, dev, n, args);
+    if(st != ROCRAND_STATUS_SUCCESS)
+        return st;
+    m_staged        = true;
+    m_staged_stream = stream;
+    if(bytes == 0)
+        return ROCRAND_STATUS_SUCCESS;
+    if(host_mode == 2)
+    {
+        if(hipMemcpyAsync(data, dev, bytes, hipMemcpyDeviceToHost, stream) != hipSuccess
+           || hipStreamSynchronize(stream) != hipSuccess)
+            return ROCRAND_STATUS_INTERNAL_ERROR;
+        return ROCRAND_STATUS_SUCCESS;
+    }
+    if(hipMemcpyAsync(m_pinned, dev, bytes, hipMemcpyDeviceToHost, stream) != hipSuccess)
         return ROCRAND_STATUS_INTERNAL_ERROR;
-    if(hipStreamSynchronize(stream) != hipSuccess)
-        return ROCRAND_STATUS_INTERNAL_ERROR;
+    host_copy* job = new(std::nothrow) host_copy{data, m_pinned, bytes};
+    if(job == nullptr)
+        return ROCRAND_STATUS_ALLOCATION_FAILED;
+    if(hipLaunchHostFunc(stream, &host_copy::run, job) != hipSuccess)
+    {
+        delete job;
+        return ROCRAND_STATUS_LAUNCH_FAILURE;
+    }
     return ROCRAND_STATUS_SUCCESS;
 }
 

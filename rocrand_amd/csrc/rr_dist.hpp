@@ -157,7 +157,7 @@ RR_DIST(word_family::w32, out_kind::normal_f32, 2, 2, float)
 RR_DIST(word_family::w32, out_kind::lognormal_f32, 2, 2, float)
 {
     float r0, r1;
-    box_muller((uint32_t)in[0], (uint32_t)in[1], r0, r1);
+    box_muller_stepwise((uint32_t)in[0], (uint32_t)in[1], r0, r1);
     out[0] = expf_acc(__fadd_rn(p.mf, __fmul_rn(r0, p.sf)));
     out[1] = expf_acc(__fadd_rn(p.mf, __fmul_rn(r1, p.sf)));
 } };
@@ -290,7 +290,7 @@ RR_DIST(word_family::w64, out_kind::lognormal_f32, 1, 2, float)
 {
     float r0, r1;
     const uint64_t v = (uint64_t)in[0];
-    box_muller((uint32_t)v, (uint32_t)(v >> 32), r0, r1);
+    box_muller_stepwise((uint32_t)v, (uint32_t)(v >> 32), r0, r1);
     out[0] = expf_acc(__fadd_rn(p.mf, __fmul_rn(r0, p.sf)));
     out[1] = expf_acc(__fadd_rn(p.mf, __fmul_rn(r1, p.sf)));
 } };
@@ -373,7 +373,7 @@ RR_DIST(word_family::w64, out_kind::poisson, 1, 1, uint32_t)
     RR_DIST(F, out_kind::lognormal_f32, 2, 2, float)                                             \
     {                                                                                            \
         float r0, r1;                                                                            \
-        mrg_box_muller<M>((uint32_t)in[0], (uint32_t)in[1], r0, r1);                             \
+        mrg_box_muller_stepwise<M>((uint32_t)in[0], (uint32_t)in[1], r0, r1);                             \
         out[0] = expf_acc(__fadd_rn(p.mf, __fmul_rn(r0, p.sf)));                                     \
         out[1] = expf_acc(__fadd_rn(p.mf, __fmul_rn(r1, p.sf)));                                     \
     } };                                                                                         \

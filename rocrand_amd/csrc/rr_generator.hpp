@@ -133,8 +133,17 @@ protected:
     bool                   pseudo_order_ok(rocrand_ordering o) const;
 
     rr::poisson_cache m_poisson;
+    // Host generators: device staging for the kernel's output, and (stream-ordered host
+    // generators) pinned staging that a host-function node copies into the caller's buffer.
+    // Both are reused by later calls, whose work is ordered after the earlier calls' on
+    // m_staged_stream; a regrow or a change of stream waits for that stream first.
     void*             m_staging       = nullptr;
     size_t            m_staging_bytes = 0;
+    void*             m_pinned        = nullptr;
+    size_t            m_pinned_bytes  = 0;
+    hipStream_t       m_staged_stream = nullptr;
+    bool              m_staged        = false;
+    rocrand_status    stage_host(void* data, size_t bytes, void*& dev);
 };
 
 namespace rr

@@ -20,6 +20,13 @@
 
 namespace rr
 {
+// The device copies live in module-scope device arrays (one instance per GPU), filled by
+// a copy on a device's first use: no device allocation is ever made for them.
+__device__ uint32_t d_mrg32_step[64 * 18], d_mrg32_sub[64 * 18], d_mrg31_step[64 * 18], d_mrg31_sub[64 * 18];
+__device__ uint32_t d_mrg32_subr[(MRG_RADIX_LO + MRG_RADIX_HI) * MRG_RADIX_WORDS];
+__device__ uint32_t d_mrg31_subr[(MRG_RADIX_LO + MRG_RADIX_HI) * MRG_RADIX_WORDS];
+__device__ uint32_t d_xorwow_jump[32 * 800], d_xorwow_seq[32 * 800], d_lfsr_jump[32 * 512], d_lfsr_seq[32 * 512];
+
 namespace
 {
 
@@ -202,13 +209,16 @@ device_cache& cache()
     return c;
 }
 
-hipError_t upload(const std::vector<uint32_t>& v, const uint32_t** out)
+// Copies a host table into the current device's instance of `symbol`.
+hipError_t upload(const std::vector<uint32_t>& v, const void* symbol, size_t symbol_bytes, const uint32_t** out)
 {
+    if(v.size() * 4 != symbol_bytes)
+        return hipErrorInvalidValue;
     void*      p   = nullptr;
-    hipError_t err = hipMalloc(&p, v.size() * 4);
+    hipError_t err = hipGetSymbolAddress(&p, symbol);
     if(err != hipSuccess)
         return err;
-    err = hipMemcpy(p, v.data(), v.size() * 4, hipMemcpyHostToDevice);
+    err  = hipMemcpy(p, v.data(), symbol_bytes, hipMemcpyHostToDevice);
     *out = static_cast<const uint32_t*>(p);
     return err;
 }
@@ -229,16 +239,19 @@ hipError_t get_device_tables(device_tables& t)
     {
         const host_tables& h = host();
         device_tables      d{};
-        if((err = upload(h.mrg32_step, &d.mrg32_step)) != hipSuccess) return err;
-        if((err = upload(h.mrg32_sub, &d.mrg32_sub)) != hipSuccess) return err;
-        if((err = upload(h.mrg31_step, &d.mrg31_step)) != hipSuccess) return err;
-        if((err = upload(h.mrg31_sub, &d.mrg31_sub)) != hipSuccess) return err;
-        if((err = upload(h.mrg32_subr, &d.mrg32_subr)) != hipSuccess) return err;
-        if((err = upload(h.mrg31_subr, &d.mrg31_subr)) != hipSuccess) return err;
-        if((err = upload(h.xorwow_jump, &d.xorwow_jump)) != hipSuccess) return err;
-        if((err = upload(h.xorwow_seq, &d.xorwow_seq)) != hipSuccess) return err;
-        if((err = upload(h.lfsr_jump, &d.lfsr_jump)) != hipSuccess) return err;
-        if((err = upload(h.lfsr_seq, &d.lfsr_seq)) != hipSuccess) return err;
+#define RR_UPLOAD(name) \
+    if((err = upload(h.name, HIP_SYMBOL(d_##name), sizeof(d_##name), &d.name)) != hipSuccess) return err;
+        RR_UPLOAD(mrg32_step)
+        RR_UPLOAD(mrg32_sub)
+        RR_UPLOAD(mrg31_step)
+        RR_UPLOAD(mrg31_sub)
+        RR_UPLOAD(mrg32_subr)
+        RR_UPLOAD(mrg31_subr)
+        RR_UPLOAD(xorwow_jump)
+        RR_UPLOAD(xorwow_seq)
+        RR_UPLOAD(lfsr_jump)
+        RR_UPLOAD(lfsr_seq)
+#undef RR_UPLOAD
         c.t[dev]     = d;
         c.ready[dev] = true;
     }

@@ -50,6 +50,21 @@ STATUS = {
 }
 
 _lib = None
+_hiplib = None
+
+
+def _hip():
+    """The HIP runtime librocrand.so links (for stream synchronization of host generators)."""
+    global _hiplib
+    if _hiplib is None:
+        _hiplib = ctypes.CDLL("libamdhip64.so", mode=ctypes.RTLD_GLOBAL)
+        _hiplib.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+    return _hiplib
+
+
+def _hip_check(err):
+    if err != 0:
+        raise RuntimeError(f"HIP error {err}")
 
 
 class _uint4(ctypes.Structure):
@@ -200,9 +215,17 @@ class RNG:
     """Base of PRNG and QRNG: one rocrand_generator handle."""
 
     def __init__(self, rngtype, offset=None, stream=None, is_host=False):
+        """is_host=True: rocrand_create_generator_host (values written in stream order; each
+        call here then synchronizes the stream, as the reference wrapper does);
+        is_host="blocking": rocrand_create_generator_host_blocking."""
         L = load()
         self._gen = ctypes.c_void_p()
-        create = L.rocrand_create_generator_host_blocking if is_host else L.rocrand_create_generator
+        if is_host == "blocking":
+            create = L.rocrand_create_generator_host_blocking
+        elif is_host:
+            create = L.rocrand_create_generator_host
+        else:
+            create = L.rocrand_create_generator
         check_rocrand(create(ctypes.byref(self._gen), rngtype))
         self.is_host = is_host
         self.rngtype = rngtype
@@ -242,6 +265,13 @@ class RNG:
         check_rocrand(load().rocrand_set_stream(self._gen, raw or None))
         self._stream = stream
 
+    def _done(self):
+        """A stream-ordered host generator has written the values once its stream drains
+        (python/rocrand/rocrand/rocrand.py:215-231 synchronizes there too)."""
+        if self.is_host is True:
+            raw = getattr(self._stream, "cuda_stream", self._stream)
+            _hip_check(_hip().hipStreamSynchronize(ctypes.c_void_p(raw or None)))
+
     def set_ordering(self, order):
         check_rocrand(load().rocrand_set_ordering(self._gen, order))
 
@@ -257,6 +287,7 @@ class RNG:
               "int32": L.rocrand_generate, "uint64": L.rocrand_generate_long_long,
               "int64": L.rocrand_generate_long_long}[dt]
         check_rocrand(fn(self._gen, p, n))
+        self._done()
 
     def uniform(self, ary, size=None):
         p, n, dt = _ptr_len(ary, size)
@@ -264,30 +295,36 @@ class RNG:
         fn = {"float32": L.rocrand_generate_uniform, "float64": L.rocrand_generate_uniform_double,
               "float16": L.rocrand_generate_uniform_half}[dt]
         check_rocrand(fn(self._gen, p, n))
+        self._done()
 
     def normal(self, ary, mean, stddev, size=None):
         p, n, dt = _ptr_len(ary, size)
         L = load()
         if dt == "float16":
             check_rocrand(L.rocrand_generate_normal_half(self._gen, p, n, _half_bits(mean), _half_bits(stddev)))
+            self._done()
             return
         fn = {"float32": L.rocrand_generate_normal, "float64": L.rocrand_generate_normal_double}[dt]
         check_rocrand(fn(self._gen, p, n, mean, stddev))
+        self._done()
 
     def lognormal(self, ary, mean, stddev, size=None):
         p, n, dt = _ptr_len(ary, size)
         L = load()
         if dt == "float16":
             check_rocrand(L.rocrand_generate_log_normal_half(self._gen, p, n, _half_bits(mean), _half_bits(stddev)))
+            self._done()
             return
         fn = {"float32": L.rocrand_generate_log_normal, "float64": L.rocrand_generate_log_normal_double}[dt]
         check_rocrand(fn(self._gen, p, n, mean, stddev))
+        self._done()
 
     def poisson(self, ary, lmbd, size=None):
         p, n, dt = _ptr_len(ary, size)
         if dt not in ("uint32", "int32"):
             raise RocRandError(103)
         check_rocrand(load().rocrand_generate_poisson(self._gen, p, n, lmbd))
+        self._done()
 
 
 class PRNG(RNG):

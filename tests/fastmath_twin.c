@@ -8,6 +8,15 @@
  * Prints, for every STRIDE-th float of each domain, the number of results that differ
  * from the correctly rounded one (the long double libm result rounded to float):
  *   log  [2^-32, 1]   sin, cos  (0, 7]   exp  (-105, 90)
+ *
+ * usage: fastmath_twin bm N [mrg]
+ * The Box-Muller pair of box_muller_uv (one double expression rounded once) over N
+ * pseudo-random word pairs (32-bit words, or MRG32k3a values in [1, m1] with "mrg"):
+ *   bm N  COUNT_BEYOND_2ULP_OF_HOST  MAX_ULP_TO_HOST  MAX_ERR_MILLIULP  HOST_MAX_ERR_MILLIULP
+ * where the host is the reference's float pipeline over this libm (sqrtf(-2 logf(u)),
+ * sinf(v) s, cosf(v) s) and the error is against the long double evaluation.  The
+ * device's float reciprocal square root is emulated by a correctly rounded one: one
+ * Newton step leaves the radius insensitive to the seed's last bits.
  */
 #include <math.h>
 #include <stdint.h>
@@ -71,8 +80,108 @@ static float expf_acc(float xf)
     return xf != xf ? xf : y;
 }
 
+static void box_muller_uv(float u, float v, float* r0, float* r1)
+{
+    const uint32_t ix = fbits(u), tmp = ix - LOGTAB_OFF;
+    const uint32_t i = (tmp >> (23 - LOGTAB_BITS)) & ((1u << LOGTAB_BITS) - 1u);
+    const int k = (int32_t)tmp >> 23;
+    const float z = bitsf(ix - (tmp & 0xff800000u));
+    const double r = fma((double)z, LOGTAB[i][0], -1.0), r2 = r * r;
+    double q = fma(r, -0.25, 1.0 / 3);
+    q = fma(q, r, -0.5);
+    const double lnu = fma((double)k, LN2, LOGTAB[i][1]) + fma(q, r2, r);
+    const double x = -2.0 * lnu;
+    const double y = (double)(1.0f / sqrtf(fmaxf((float)x, 0x1p-126f)));
+    const double t = x * y, h = fma(-t, y, 1.0), s = fma(t * h, 0.5, t);
+    const double vd = v, qv = rint(vd * SC_INV_STEP);
+    const int kv = (int)qv & ((1 << SCTAB_BITS) - 1);
+    double a = fma(-qv, SC_STEP_HI, vd);
+    a = fma(-qv, SC_STEP_LO, a);
+    const double a2 = a * a;
+    const double sa = fma(a * a2, fma(a2, 1.0 / 120, -1.0 / 6), a);
+    const double ca = fma(a2, fma(a2, 1.0 / 24, -0.5), 1.0);
+    const double sn = fma(SCTAB[kv][0], ca, SCTAB[kv][1] * sa);
+    const double cs = fma(SCTAB[kv][1], ca, -(SCTAB[kv][0] * sa));
+    *r0 = (float)(sn * s);
+    *r1 = (float)(cs * s);
+}
+
+static int64_t ford(float x)
+{
+    const int32_t i = (int32_t)fbits(x);
+    return i < 0 ? (int64_t)INT32_MIN - i : i;
+}
+
+/* |got - want| in ulps of want's binade, against a long double reference */
+static double ulp_err(float got, long double want)
+{
+    const float w = (float)want;
+    const float sp = nextafterf(fabsf(w), INFINITY) - fabsf(w);
+    return (double)(fabsl((long double)got - want) / sp);
+}
+
+static int box_muller_sample(long n, int mrg)
+{
+    uint64_t st = 0x2545F4914F6CDD1Dull;
+    long beyond = 0;
+    int64_t maxd = 0;
+    double maxe = 0.0, maxh = 0.0;
+    for(long i = 0; i < n + 2; i++)
+    {
+        uint32_t w[2];
+        for(int j = 0; j < 2; j++)
+        {
+            st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+            w[j] = (uint32_t)(st >> 11);
+            if(mrg)
+                w[j] = 1u + (uint32_t)((st >> 8) % 4294967087ull);
+        }
+        if(i == n) /* u = 1: the radius is -0 */
+            w[0] = mrg ? 4294967087u : 0xffffffffu;
+        if(i == n + 1) /* the smallest u */
+            w[0] = mrg ? 1u : 0u;
+        float u, v;
+        if(mrg)
+        {
+            u = (float)((double)w[0] * 2.3283065498378288e-10);
+            v = (float)((double)w[1] * 2.3283065498378288e-10) * 6.2831855f;
+        }
+        else
+        {
+            u = 2.3283064e-10f + (float)w[0] * 2.3283064e-10f;
+            v = 1.46291807e-09f + (float)w[1] * 1.46291807e-09f;
+        }
+        float d0, d1;
+        box_muller_uv(u, v, &d0, &d1);
+        const float hs = sqrtf(-2.0f * logf(u));
+        const float h[2] = {sinf(v) * hs, cosf(v) * hs};
+        const long double S = sqrtl(-2.0L * logl((long double)u));
+        const long double e[2] = {sinl((long double)v) * S, cosl((long double)v) * S};
+        const float d[2] = {d0, d1};
+        for(int j = 0; j < 2; j++)
+        {
+            if(signbit(d[j]) != signbit(h[j]))
+                beyond++;
+            int64_t dd = ford(d[j]) - ford(h[j]);
+            dd = dd < 0 ? -dd : dd;
+            beyond += dd > 2;
+            maxd = dd > maxd ? dd : maxd;
+            if(e[j] != 0.0L)
+            {
+                const double er = ulp_err(d[j], e[j]), eh = ulp_err(h[j], e[j]);
+                maxe = er > maxe ? er : maxe;
+                maxh = eh > maxh ? eh : maxh;
+            }
+        }
+    }
+    printf("bm %ld %ld %ld %ld %ld\n", 2 * (n + 2), beyond, (long)maxd, (long)(maxe * 1000), (long)(maxh * 1000));
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
+    if(argc > 2 && strcmp(argv[1], "bm") == 0)
+        return box_muller_sample(atol(argv[2]), argc > 3 && strcmp(argv[3], "mrg") == 0);
     const int64_t stride = argc > 1 ? atoll(argv[1]) : 1;
     long nlog = 0, blog = 0, ntrig = 0, bsin = 0, bcos = 0, nexp = 0, bexp = 0;
     for(int64_t b = fbits(ldexpf(1.0f, -32)); b <= fbits(1.0f); b += stride)

@@ -3,11 +3,13 @@
 Integers, uniforms and Poisson are compared bit-exactly.  The floating-point bars
 follow from where the device and the oracle (host libm) may legitimately differ:
 
-* Box-Muller normals, float/double: the host's float logf/sinf/cosf are not
-  correctly rounded (~1% of calls return the other neighbour) and the device's
-  are, so a pair can sit up to 3 ulp apart.  Bound: 3 ulp of the variate v,
-  scaled by |stddev|, plus the two roundings of mean + v*stddev; at most
-  1 + 2e-4*n values beyond 2 ulp.
+* Box-Muller normals, float: the device evaluates sin(v) sqrt(-2 ln u) as one double
+  expression rounded once (within 0.51 ulp of exact); the host's float pipeline
+  (logf, sqrtf, sinf/cosf, product) is within 2.4 ulp of exact.  Bar: at most 2 ulp
+  (plain ulp distance) at mean 0, stddev 1; otherwise 2 ulp of the variate v carried
+  through the affine mean + v * stddev (its two roundings may then differ by one ulp each).
+* Box-Muller normals, double: the device's double log/sin/cos are within one ulp, as
+  are the host's; bar as for float (2 ulp of the variate, plain 2 ulp at (0, 1)).
 * Half normals: the reference rounds the Box-Muller variate to half before the
   affine (rocrand_normal.h:108-135 host branch).  A one-float-ulp difference next
   to a half rounding boundary moves that intermediate by one half ulp, which the
@@ -16,9 +18,16 @@ follow from where the device and the oracle (host libm) may legitimately differ:
 * Inverse-error-function normals (Sobol, ThreeFry2x32 doubles, Poisson above
   lambda 4000): rocrand_normal.h:174-221 evaluates -tt1 + sqrt(tt1^2 - tt2) with
   tt1 ~ 4.33, which cancels near the median; a one-ulp difference in the log moves
-  tt1's rounding by one ulp and the output by up to ~ulp(tt1) / |v|.  Bound:
-  3 ulp(v) + 4 ulp(4.33) / |v| (times |stddev|), with 99% of values bit-equal.
-* Log-normals are judged on log(out), where exp's rounding adds 2 ulp relative.
+  tt1 by one ulp and v^2 by up to 2 ulp(tt1), so v by at most
+  min(2 ulp(tt1) / |v|, sqrt(2 ulp(tt1))).  Bound: 3 ulp(v) plus that term (times
+  |stddev|): in ulps of v away from the median, with a bounded absolute floor near it;
+  99% of values bit-equal.
+* Log-normals: exp magnifies the variate's error by |log out| / ulp, so they are judged
+  on the variate log(out): 2 ulp of the variate through the affine plus exp's rounding
+  (one ulp relative on either side).  The plain ulp distance of the outputs is reported
+  by the caller; at most 1 + 3e-4 n values may exceed 2 ulp (the device takes the
+  reference's own float steps for log-normal variates, which equal the host's bit for
+  bit wherever the host libm rounds correctly; measured 1.1e-4 n on the host twin).
 """
 import numpy as np
 
@@ -54,17 +63,25 @@ def check_box_muller(kind, got, want, mean, sd):
         assert bad.size == 0, f"half values beyond the bound at {bad[:5]}: got {got[bad[:3]]} want {want[bad[:3]]}"
         return
     ftype = got.dtype.type
+    if not log and mean == 0.0 and sd == 1.0:
+        d = ulp_distance(got, want)
+        worst = int(np.argmax(d))
+        assert d[worst] <= 2, f"{int(d[worst])} ulp at {worst}: got {got[worst]!r} want {want[worst]!r}"
+        return
     ulp_v = _affine_bound(v, sd, ftype)
     if log:
-        ulp_o = 2 * np.finfo(ftype).eps * np.maximum(np.abs(zw), 1.0)
+        # exp's rounding on either side: one ulp relative, i.e. eps in log space
+        ulp_o = 2 * np.finfo(ftype).eps + np.spacing(np.abs(zw).astype(ftype)).astype(np.float64)
+        d = ulp_distance(got, want)
+        over = np.count_nonzero(d > 2)
+        assert over <= 1 + 3e-4 * d.size, f"{over} of {d.size} log-normal values beyond 2 ulp"
     else:
         ulp_o = (np.spacing(np.abs(want)).astype(np.float64)
                  + np.spacing(np.abs(v * sd).astype(ftype)).astype(np.float64))
     err = np.abs(zg - zw)
-    worst = int(np.argmax(err - 3 * ulp_v - ulp_o))
-    assert np.all(err <= 3 * ulp_v + ulp_o), f"beyond 3 ulp at {worst}: got {got[worst]!r} want {want[worst]!r}"
-    over = np.count_nonzero(err > 2 * ulp_v + ulp_o)
-    assert over <= 1 + 2e-4 * err.size, f"{over} of {err.size} values at 3 ulp"
+    lim = 2 * ulp_v + ulp_o
+    worst = int(np.argmax(err - lim))
+    assert np.all(err <= lim), f"beyond 2 ulp of the variate at {worst}: got {got[worst]!r} want {want[worst]!r}"
 
 
 def check_erfinv(kind, got, want, mean, sd):
@@ -76,9 +93,8 @@ def check_erfinv(kind, got, want, mean, sd):
     zg = np.log(got.astype(np.float64)) if log else got.astype(np.float64)
     zw = np.log(want.astype(np.float64)) if log else want.astype(np.float64)
     v = (zw - mean) / sd
-    r = np.maximum(np.abs(v) / np.sqrt(2.0), 1e-30)
-    bound = (3 * np.spacing(np.abs(v).astype(ftype)).astype(np.float64)
-             + 4 * c1_ulp * np.sqrt(2.0) / (2 * r)) * abs(sd)
+    cancel = np.minimum(2 * c1_ulp / np.maximum(np.abs(v), 1e-300), np.sqrt(2 * c1_ulp))
+    bound = (3 * np.spacing(np.abs(v).astype(ftype)).astype(np.float64) + cancel) * abs(sd)
     slack = 2 * np.spacing(np.abs(zw).astype(ftype)).astype(np.float64)
     if got.dtype == np.float16:
         hs = np.spacing(np.abs(want).astype(np.float16)).astype(np.float64)
@@ -86,8 +102,6 @@ def check_erfinv(kind, got, want, mean, sd):
     err = np.abs(zg - zw)
     bad = np.nonzero(~(err <= bound + slack))[0]
     assert bad.size == 0, f"{bad.size} values beyond the bound, first {bad[:3]}: got {got[bad[:3]]} want {want[bad[:3]]}"
-    if kind.endswith("f64") and log:
-        return  # the device's double exp is not correctly rounded: last-bit differences
     same = np.count_nonzero(got.view(f"u{got.dtype.itemsize}") == want.view(f"u{want.dtype.itemsize}"))
     # the 99% rate is a large-sample figure: a few last-bit differences may fall in a
     # sample of a few dozen values

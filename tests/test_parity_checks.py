@@ -55,3 +55,28 @@ def test_exact_bar():
     bad[5] = np.nextafter(bad[5], np.float32(2))
     with pytest.raises(AssertionError):
         check("uniform_f32", bad, want)
+
+
+@pytest.mark.parametrize("kind", ["normal_f32", "normal_f64"])
+def test_box_muller_bar_is_two_ulp(kind):
+    # at mean 0, stddev 1 the bar is the plain ulp distance: 2 ulp passes, 3 fails
+    want = _oracle(O.PHILOX4_32_10, kind, 4000, 0.0, 1.0, seed=8)
+    two = want.copy()
+    for _ in range(2):
+        two[10] = np.nextafter(two[10], np.inf, dtype=want.dtype)
+    check(kind, two, want, (0.0, 1.0))
+    three = two.copy()
+    three[10] = np.nextafter(three[10], np.inf, dtype=want.dtype)
+    with pytest.raises(AssertionError):
+        check(kind, three, want, (0.0, 1.0))
+
+
+def test_erfinv_bar_is_bounded_near_the_median():
+    # near v = 0 the cancellation term is capped at sqrt(2 ulp(4.33)) (~1e-3 for float):
+    # a value 0.01 away from the oracle's fails however close to the median it lies
+    want = _oracle(O.SOBOL32, "normal_f32", 20000, 0.0, 1.0)
+    i = int(np.argmin(np.abs(want)))
+    bad = want.copy()
+    bad[i] = bad[i] + np.float32(0.01)
+    with pytest.raises(AssertionError):
+        check("normal_f32", bad, want, (0.0, 1.0), erfinv=True)
