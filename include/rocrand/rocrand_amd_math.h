@@ -39,8 +39,6 @@ constexpr float  R_SQRT2          = 1.4142135f;
 //   logf_tab (Box-Muller radius, u a positive normal float): u = 2^k z with
 //        z in [0.699, 1.398), ln u = k ln 2 - ln(1/c_i) + log1p(z/c_i - 1) over the 64-entry
 //        table of rocrand_amd_mathtab.h, |z/c_i - 1| < 2^-7, series to r^6 (relative error < 2^-44);
-//   log (erfinv argument, any float): x = 2^e m, m in [sqrt(1/2), sqrt(2)),
-//        ln m = 2 atanh(f), f = (m-1)/(m+1), |f| < 0.172, series to f^21;
 //   sin/cos for x in [0, 8): reduction by pi/32 (two parts, fused), sin/cos(k pi/32) from
 //        rocrand_amd_mathtab.h and Taylor series to r^7 / r^6 on |r| <= pi/64;
 //   expf_acc: reduction by ln2/32 (two parts), 2^(j/32) from rocrand_amd_mathtab.h, Taylor series
@@ -51,8 +49,6 @@ constexpr float  R_SQRT2          = 1.4142135f;
 // (sin, cos) and every float in (-105, 90) but two (exp).
 // So each differs from the host libm's float result by at most the host's own error
 // (< 1 ulp).
-constexpr double RR_LN2_HI  = 6.93147180369123816490e-01; // ln 2, top 32 bits
-constexpr double RR_LN2_LO  = 1.90821492927058770002e-10; // ln 2 - RR_LN2_HI
 constexpr double RR_LN2     = 6.93147180559945286227e-01; // ln 2 rounded to double
 
 // ln u for a positive normal float u (the Box-Muller uniforms lie in [2^-32, 1]).
@@ -73,41 +69,6 @@ __device__ __forceinline__ float logf_tab(float u)
     p                  = __builtin_fma(p, r, -0.5);
     const double hi    = __builtin_fma((double)k, RR_LN2, t.y);
     return (float)(hi + __builtin_fma(p, r2, r));
-}
-
-__device__ __forceinline__ double log_dbl(double x)
-{
-    ROCRAND_AMD_FP_EXACT
-    int    e = __builtin_amdgcn_frexp_exp(x);
-    double m = __builtin_amdgcn_frexp_mant(x); // [0.5, 1)
-    if(m < 0.70710678118654752440)
-    {
-        m = m * 2.0;
-        e = e - 1;
-    }
-    const double d  = m + 1.0;
-    double       r  = __builtin_amdgcn_rcp(d);
-    r               = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
-    const double f  = (m - 1.0) * r;
-    const double f2 = f * f;
-    double       p  = 1.0 / 21;
-    p               = __builtin_fma(p, f2, 1.0 / 19);
-    p               = __builtin_fma(p, f2, 1.0 / 17);
-    p               = __builtin_fma(p, f2, 1.0 / 15);
-    p               = __builtin_fma(p, f2, 1.0 / 13);
-    p               = __builtin_fma(p, f2, 1.0 / 11);
-    p               = __builtin_fma(p, f2, 1.0 / 9);
-    p               = __builtin_fma(p, f2, 1.0 / 7);
-    p               = __builtin_fma(p, f2, 1.0 / 5);
-    p               = __builtin_fma(p, f2, 1.0 / 3);
-    const double tf = 2.0 * f;
-    const double lm = __builtin_fma(tf * f2, p, tf);
-    const double y  = __builtin_fma((double)e, RR_LN2_HI, __builtin_fma((double)e, RR_LN2_LO, lm));
-    if(x == 0.0)
-        return -__builtin_inf();
-    if(!(x > 0.0) || x == __builtin_inf())
-        return x < 0.0 ? __builtin_nan("") : x; // NaN, negative -> NaN; +inf -> +inf
-    return y;
 }
 
 // sin and cos of x in [0, 8), rounded to float: x = q pi/32 + r (two-part reduction,
@@ -149,7 +110,7 @@ __device__ __forceinline__ float sqrtf_radius(float x)
     return r;
 }
 
-__device__ __forceinline__ float logf_acc(float x) { return (float)log_dbl((double)x); }
+
 
 // exp of a float, rounded to float: x = (32 e + j) ln2/32 + r, exp x = 2^e 2^(j/32) exp(r)
 // with the 2^(j/32) table of rocrand_amd_mathtab.h.  Arguments are clamped to [-110, 100]
@@ -171,6 +132,165 @@ __device__ __forceinline__ float expf_acc(float xf)
     const double t  = rr_exptab[k & ((1 << RR_EXPTAB_BITS) - 1)];
     const float  y  = (float)__builtin_amdgcn_ldexp(__builtin_fma(t, p, t), k >> RR_EXPTAB_BITS);
     return xf != xf ? xf : y;
+}
+
+// ---------------------------------------------------------------- double functions
+// The double transforms' log, exp, sin and cos, evaluated in double-double arithmetic
+// (Dekker's exact sum and product) and rounded once: each result is within
+// 2^-66 (relative) of the exact value before that rounding, so it is the correctly rounded
+// double except for arguments within that distance of a rounding boundary.  The
+// reference's host path takes these steps through the host libm, which rounds them
+// correctly but for rare hard cases; evaluated this way the device reproduces its
+// intermediate values.  Host and device compile the same code (the host twin of the
+// tests checks them against quad precision).
+__host__ __device__ __forceinline__ void rr_two_sum(double a, double b, double& s, double& e)
+{
+    ROCRAND_AMD_FP_EXACT
+    s               = a + b;
+    const double bb = s - a;
+    e               = (a - (s - bb)) + (b - bb);
+}
+// |a| >= |b| (or a = 0)
+__host__ __device__ __forceinline__ void rr_fast_two_sum(double a, double b, double& s, double& e)
+{
+    ROCRAND_AMD_FP_EXACT
+    s = a + b;
+    e = b - (s - a);
+}
+__host__ __device__ __forceinline__ void rr_two_prod(double a, double b, double& p, double& e)
+{
+    ROCRAND_AMD_FP_EXACT
+    p = a * b;
+    e = __builtin_fma(a, b, -p);
+}
+__host__ __device__ __forceinline__ double rr_bits_to_double(uint64_t b)
+{
+    double x;
+    __builtin_memcpy(&x, &b, sizeof(x));
+    return x;
+}
+__host__ __device__ __forceinline__ uint64_t rr_double_to_bits(double x)
+{
+    uint64_t b;
+    __builtin_memcpy(&b, &x, sizeof(b));
+    return b;
+}
+
+// ln x: x = 2^k m, m in [sqrt(1/2), sqrt(2)), c the centre of m's table interval
+// (rocrand_amd_mathtab.h), r = m/c - 1 exact as a double-double (|r| < 2^-8), and
+// ln x = k ln 2 - ln(1/c) + log1p(r) with log1p to r^10.
+__host__ __device__ __forceinline__ double log_cr(double x)
+{
+    ROCRAND_AMD_FP_EXACT
+    if(!(x > 0.0) || x == __builtin_inf())
+        return x == 0.0 ? -__builtin_inf() : (x > 0.0 ? x : __builtin_nan(""));
+    int kadj = 0;
+    if(x < 0x1p-1022)
+    {
+        x *= 0x1p54; // subnormal: scale into the normal range
+        kadj = -54;
+    }
+    const uint64_t ix  = rr_double_to_bits(x);
+    const uint32_t tmp = (uint32_t)(ix >> 32) - RR_DLOG_OFF;
+    const int      ke  = (int)tmp >> 20;
+    const double   k   = (double)(ke + kadj);
+    const double   m   = rr_bits_to_double(ix - ((uint64_t)(int64_t)ke << 52));
+    const double*  e   = rr_dlog_tab[(tmp >> 13) & 127u];
+    double         p, pe, rh, rl;
+    rr_two_prod(m, e[0], p, pe);
+    rr_two_sum(p - 1.0, pe, rh, rl); // p - 1 is exact: p is within 2^-7 of 1
+    const double r2  = rh * rh;
+    const double r2e = __builtin_fma(rh, rh, -r2);
+    double       q   = __builtin_fma(rh, -1.0 / 10, 1.0 / 9);
+    q                = __builtin_fma(q, rh, -1.0 / 8);
+    q                = __builtin_fma(q, rh, 1.0 / 7);
+    q                = __builtin_fma(q, rh, -1.0 / 6);
+    q                = __builtin_fma(q, rh, 1.0 / 5);
+    q                = __builtin_fma(q, rh, -1.0 / 4);
+    q                = __builtin_fma(q, rh, 1.0 / 3);
+    // log1p(r) = r - r^2/2 + r^3 q(r); the tail's share of r^2 / 2 and r^3 / 3 folded below
+    double a, ae, b, be, c, ce;
+    rr_two_sum(k * RR_DLN2_HI, e[1], a, ae);
+    rr_two_sum(a, rh, b, be);
+    rr_two_sum(b, -0.5 * r2, c, ce);
+    const double tail = (rl - 0.5 * r2e - rh * rl) + r2 * (rh * q + rl);
+    const double lo   = (ae + be + ce) + (k * RR_DLN2_MI + e[2]) + tail;
+    return c + lo;
+}
+
+// e^x: x = (64 k + j) ln2/64 + r (|r| <= ln2/128, the product with the head of ln2/64
+// exact), e^x = 2^k 2^(j/64) (1 + expm1(r)) with 2^(j/64) a double-double and expm1 to r^8.
+__host__ __device__ __forceinline__ double exp_cr(double x)
+{
+    ROCRAND_AMD_FP_EXACT
+    if(x != x)
+        return x;
+    if(x > 709.782712893384)
+        return __builtin_inf();
+    if(x < -745.1332191019412)
+        return 0.0;
+    const double qd = __builtin_rint(x * RR_DEXP_INV_STEP);
+    const int    q  = (int)qd;
+    double       rh, rl;
+    rr_two_sum(__builtin_fma(-qd, RR_DEXP_STEP_HI, x), -qd * RR_DEXP_STEP_MI, rh, rl);
+    rl               = __builtin_fma(-qd, RR_DEXP_STEP_LO, rl);
+    const double r2  = rh * rh;
+    const double r2e = __builtin_fma(rh, rh, -r2);
+    double       pp  = __builtin_fma(rh, 1.0 / 40320, 1.0 / 5040);
+    pp               = __builtin_fma(pp, rh, 1.0 / 720);
+    pp               = __builtin_fma(pp, rh, 1.0 / 120);
+    pp               = __builtin_fma(pp, rh, 1.0 / 24);
+    pp               = __builtin_fma(pp, rh, 1.0 / 6);
+    // expm1(r) = eh + el
+    const double  eh = rh;
+    const double  el = __builtin_fma(0.5, r2, (rl + (0.5 * r2e + rh * rl)) + r2 * rh * pp);
+    const double2 t  = rr_dexp_tab[q & 63];
+    double        s1, s1e, s, se;
+    rr_two_prod(t.x, eh, s1, s1e);
+    rr_fast_two_sum(t.x, s1, s, se);
+    const double lo = (se + s1e) + (t.x * el + t.y) + t.y * (eh + el);
+    return ldexp(s + lo, q >> 6);
+}
+
+// sin t and cos t for t in [0, 2 pi + 2^-48]: t = q pi/64 + r (|r| <= pi/128, the products
+// with the two heads of pi/64 exact), then the angle-addition formulas with sin(q pi/64),
+// cos(q pi/64) as double-doubles and sin r to r^9, cos r to r^10.  Where sin t (cos t) is
+// near 0 the table entry is an exact 0 and the result keeps its relative accuracy.
+__host__ __device__ __forceinline__ void sincos_cr(double t, double& sn, double& cs)
+{
+    ROCRAND_AMD_FP_EXACT
+    const double qd = __builtin_rint(t * RR_DSC_INV_STEP);
+    const int    q  = (int)qd;
+    double       rh, rl;
+    rr_two_sum(__builtin_fma(-qd, RR_DSC_STEP_HI, t), -qd * RR_DSC_STEP_MI, rh, rl);
+    rl               = __builtin_fma(-qd, RR_DSC_STEP_LO, rl);
+    const double r2  = rh * rh;
+    const double r2e = __builtin_fma(rh, rh, -r2);
+    double       ps  = __builtin_fma(r2, 1.0 / 362880, -1.0 / 5040);
+    ps               = __builtin_fma(ps, r2, 1.0 / 120);
+    ps               = __builtin_fma(ps, r2, -1.0 / 6);
+    double pc        = __builtin_fma(r2, -1.0 / 3628800, 1.0 / 40320);
+    pc               = __builtin_fma(pc, r2, -1.0 / 720);
+    pc               = __builtin_fma(pc, r2, 1.0 / 24);
+    // sin r = rh + sl, cos r = 1 + ch + cl
+    const double  sl = __builtin_fma(r2 * rh, ps, __builtin_fma(-0.5 * r2, rl, rl));
+    const double  ch = -0.5 * r2;
+    const double  cl = __builtin_fma(r2 * r2, pc, -(0.5 * r2e + rh * rl));
+    const double* T  = rr_dsc_tab[q];
+    const double  S = T[0], Sl = T[1], C = T[2], Cl = T[3];
+    double        p1, p1e, p2, p2e, a, ae, b, be;
+    // sin t = S cos r + C sin r
+    rr_two_prod(C, rh, p1, p1e);
+    rr_two_prod(S, ch, p2, p2e);
+    rr_two_sum(S, p1, a, ae);
+    rr_two_sum(a, p2, b, be);
+    sn = b + (((ae + be) + (p1e + p2e)) + ((S * cl + C * sl) + (Sl * (1.0 + ch) + Cl * (rh + sl))));
+    // cos t = C cos r - S sin r
+    rr_two_prod(-S, rh, p1, p1e);
+    rr_two_prod(C, ch, p2, p2e);
+    rr_two_sum(C, p1, a, ae);
+    rr_two_sum(a, p2, b, be);
+    cs = b + (((ae + be) + (p1e + p2e)) + ((C * cl - S * sl) + (Cl * (1.0 + ch) - Sl * (rh + sl))));
 }
 
 // rocrand_uniform.h:63-66: 2^-32 + v * 2^-32.  The product by a power of two is
@@ -244,58 +364,68 @@ __device__ __forceinline__ double mrg_f64(uint32_t v)
 }
 
 // The Box-Muller pair sin(v) sqrt(-2 ln u), cos(v) sqrt(-2 ln u) for a float u in
-// [2^-32, 1] and a float angle v in (0, 8), evaluated as one expression in double and
-// rounded to float once.  Relative error before that rounding is below 2^-36 (log1p series
-// to r^4 on |r| < 2^-7: 2^-30 relative on ln u, halved by the square root; sin series to
-// r^5 and cos to r^4 on |r| <= pi/64; a float reciprocal square root refined by one Newton
-// step), so each output is within 0.51 ulp of the exact value.  The reference's float
-// pipeline (logf, sqrtf, sinf/cosf, product: four roundings, rocrand_normal.h:51-66) lies
-// within 2.4 ulp of the exact value, so the two stay within 2 ulp of each other.
-// u = 1 gives ln u = +0 (table entry {1, 0}), x = -0 and the radius -0, as the reference's
-// sqrtf(-2.0f * logf(1.0f)); the clamp keeps the reciprocal square root finite there.
-__device__ __forceinline__ void box_muller_uv(float u, float v, float& r0, float& r1)
+// [2^-32, 1] and a float angle v in (0, 2 pi], evaluated in double and rounded to float
+// once.  qoff = 16 q selects the angle table entry for q = round(128 v / pi), which the
+// callers take from the word the angle came from (integer operations).
+//   -2 ln u: u's exponent and 6 leading mantissa bits index rr_bm_logtab (a = 2^-k / c,
+//     b = 2 ln a), r = u a - 1 (|r| < 2^-7) and -2 ln u = b + r (-2 + r (1 + r (-2/3 + r/2)))
+//     (log1p to r^4, relative error below 2^-35; u = 1 gives b = -0 and r = 0, so -0 as the
+//     reference's -2.0f * logf(1.0f));
+//   sqrt: the hardware double reciprocal square root (relative error below 2^-22) and one
+//     coupled Newton step (error below 2^-42); the radius keeps the sign of a zero;
+//   sin/cos: v = q P + a with P = pi/128 to 15 bits, a = v - q P exact in float
+//     (|a| < 0.0124), sin a to a^3 and cos a to a^4 (relative errors below 2^-32 and 2^-44),
+//     then the angle-addition formulas with sin(q P), cos(q P) from rr_bm_sctab.
+// The pair before its rounding is within 2^-30 of exact, so each output is within 0.51
+// ulp of the exact value, and within 2 ulp of the reference's float pipeline (logf, sqrtf,
+// sinf/cosf and the product: four roundings, rocrand_normal.h:51-66).  UNIT rounds
+// sin(v) s + 0 instead of sin(v) s: the output of mean + 1 * x at mean +0 (a zero is +0).
+template<bool UNIT = false>
+__device__ __forceinline__ void box_muller_dd(float u, float v, uint32_t qoff, float& r0, float& r1)
 {
     ROCRAND_AMD_FP_EXACT
-    const uint32_t ix  = __float_as_uint(u);
-    const uint32_t tmp = ix - RR_LOGTAB_OFF;
-    const uint32_t i   = (tmp >> (23 - RR_LOGTAB_BITS)) & ((1u << RR_LOGTAB_BITS) - 1u);
-    const int      k   = (int)tmp >> 23;
-    const float    z   = __uint_as_float(ix - (tmp & 0xff800000u));
-    const double2  lt  = rr_logtab[i];
-    const double   r   = __builtin_fma((double)z, lt.x, -1.0);
-    const double   r2  = r * r;
-    double         q   = __builtin_fma(r, -0.25, 1.0 / 3);
-    q                  = __builtin_fma(q, r, -0.5);
-    const double   lnu = __builtin_fma((double)k, RR_LN2, lt.y) + __builtin_fma(q, r2, r);
-    const double   x   = -2.0 * lnu;
-    const double   y   = (double)__builtin_amdgcn_rsqf(__builtin_fmaxf((float)x, 0x1p-126f));
-    const double   t   = x * y;
-    const double   h   = __builtin_fma(-t, y, 1.0);
-    const double   s   = __builtin_fma(t * h, 0.5, t);
+    const uint32_t loff = (__float_as_uint(u) >> 13) & 0x1fff0u; // 16 * (exponent * 64 + i)
+    const double2  lt   = *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(rr_bm_logtab)
+                                                         - RR_BM_EMIN * 64u * 16u + loff);
+    const double   r    = __builtin_fma((double)u, lt.x, -1.0);
+    double         p    = __builtin_fma(r, 0.5, -2.0 / 3);
+    p                   = __builtin_fma(p, r, 1.0);
+    p                   = __builtin_fma(p, r, -2.0);
+    const double   x    = __builtin_fma(p, r, lt.y);
+    const double   y    = __builtin_amdgcn_rsq(__builtin_fmax(x, 0x1p-126));
+    const double   t    = x * y;
+    const double   h    = __builtin_fma(-t, y, 1.0);
+    const double   s    = __builtin_fma(t * h, 0.5, t);
 
-    const double  vd = (double)v;
-    const double  qv = __builtin_rint(vd * RR_SC_INV_STEP);
-    const int     kv = (int)qv & ((1 << RR_SCTAB_BITS) - 1);
-    double        a  = __builtin_fma(-qv, RR_SC_STEP_HI, vd);
-    a                = __builtin_fma(-qv, RR_SC_STEP_LO, a);
-    const double2 st = rr_sctab[kv];
+    const double2 st = *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(rr_bm_sctab) + qoff);
+    const double  a  = (double)__builtin_fmaf(-(float)qoff, RR_BM_ANGLE_STEP * 0.0625f, v);
     const double  a2 = a * a;
-    const double  sa = __builtin_fma(a * a2, __builtin_fma(a2, 1.0 / 120, -1.0 / 6), a);
+    const double  sa = a * __builtin_fma(a2, -1.0 / 6, 1.0);
     const double  ca = __builtin_fma(a2, __builtin_fma(a2, 1.0 / 24, -0.5), 1.0);
     const double  sn = __builtin_fma(st.x, ca, st.y * sa);
     const double  cs = __builtin_fma(st.y, ca, -(st.x * sa));
-    r0               = (float)(sn * s);
-    r1               = (float)(cs * s);
+    if constexpr(UNIT)
+    {
+        r0 = (float)__builtin_fma(sn, s, 0.0);
+        r1 = (float)__builtin_fma(cs, s, 0.0);
+    }
+    else
+    {
+        r0 = (float)(sn * s);
+        r1 = (float)(cs * s);
+    }
 }
 
 // rocrand_normal.h:51-66: u = 2^-32 + x 2^-32, angle 2^-32 2pi + y 2^-32 2pi (rounded as
-// the reference rounds them), then the pair above.
+// the reference rounds them), then the pair above; the angle is about 2 pi (y + 1) 2^-32,
+// so q = round(y 2^-24).
+template<bool UNIT = false>
 __device__ __forceinline__ void box_muller(uint32_t x, uint32_t y, float& r0, float& r1)
 {
     ROCRAND_AMD_FP_EXACT
     const float u = uniform_f32(x);
     const float v = R_2POW32_INV_2PI + (float)y * R_2POW32_INV_2PI;
-    box_muller_uv(u, v, r0, r1);
+    box_muller_dd<UNIT>(u, v, ((y >> 20) + 8u) & ~15u, r0, r1);
 }
 
 // The same pair through the reference's own steps -- logf, sqrtf, sinf/cosf, each rounded
@@ -327,10 +457,10 @@ __device__ __forceinline__ void box_muller_double(uint32_t a, uint32_t b, uint32
     const double   u  = R_2POW53_INV_DBL + (double)v1 * R_2POW53_INV_DBL;
     const uint64_t v2 = (uint64_t)c ^ ((uint64_t)d << 21);
     const double   w  = R_2POW53_INV_DBL * 2.0 + (double)v2 * (R_2POW53_INV_DBL * 2.0);
-    const double   s  = sqrt(-2.0 * log(u));
+    const double   s  = __builtin_sqrt(-2.0 * log_cr(u));
     const double   t  = w * R_PI_DBL;
     double         sn, cs;
-    sincos(t, &sn, &cs);
+    sincos_cr(t, sn, cs);
     r0 = sn * s;
     r1 = cs * s;
 }
@@ -347,11 +477,14 @@ __device__ __forceinline__ void box_muller_half(uint32_t x16, uint32_t y16, floa
     h1 = __half2float(__float2half(cs * s));
 }
 // rocrand_normal.h:138-154
-template<int MRG>
+// The MRG outputs y in [1, m] scale by about 2^-32 (MRG32k3a) or 2^-31 (MRG31k3p), so
+// q = round(y 2^-24) or round(y 2^-23).
+template<int MRG, bool UNIT = false>
 __device__ __forceinline__ void mrg_box_muller(uint32_t x, uint32_t y, float& r0, float& r1)
 {
     ROCRAND_AMD_FP_EXACT
-    box_muller_uv(mrg_f32<MRG>(x), mrg_f32<MRG>(y) * R_2PI, r0, r1);
+    constexpr int SH = MRG == 32 ? 20 : 19;
+    box_muller_dd<UNIT>(mrg_f32<MRG>(x), mrg_f32<MRG>(y) * R_2PI, ((y >> SH) + 8u) & ~15u, r0, r1);
 }
 template<int MRG>
 __device__ __forceinline__ void mrg_box_muller_stepwise(uint32_t x, uint32_t y, float& r0, float& r1)
@@ -366,9 +499,9 @@ __device__ __forceinline__ void mrg_box_muller_double(uint32_t x, uint32_t y, do
     ROCRAND_AMD_FP_EXACT
     const double u = (double)mrg_f32<MRG>(x);
     const double v = (double)mrg_f32<MRG>(y) * 2.0;
-    const double s = sqrt(-2.0 * log(u));
+    const double s = __builtin_sqrt(-2.0 * log_cr(u));
     double       sn, cs;
-    sincos(v * R_PI_DBL, &sn, &cs);
+    sincos_cr(v * R_PI_DBL, sn, cs);
     r0 = sn * s;
     r1 = cs * s;
 }
@@ -378,14 +511,14 @@ __device__ __forceinline__ double roc_d_erfinv(double x)
     ROCRAND_AMD_FP_EXACT
     const double sgn = (x < 0.0) ? -1.0 : 1.0;
     x                = (1.0 - x) * (1.0 + x);
-    const double lnx = log(x);
+    const double lnx = log_cr(x);
     if(isnan(lnx))
         return 1.0;
     if(isinf(lnx))
         return 0.0;
     const double tt1 = 2.0 / (R_PI_DBL * 0.147) + 0.5 * lnx;
     const double tt2 = 1.0 / 0.147 * lnx;
-    return sgn * sqrt(-tt1 + sqrt(tt1 * tt1 - tt2));
+    return sgn * __builtin_sqrt(-tt1 + __builtin_sqrt(tt1 * tt1 - tt2));
 }
 __device__ __forceinline__ double normal_double_1(double p)
 {
@@ -400,7 +533,7 @@ __device__ __forceinline__ float roc_f_erfinv(float x)
     const float sgn = (x < 0.0f) ? -1.0f : 1.0f;
     x               = (1.0f - x) * (1.0f + x);
     // (1 - x)(1 + x) is a positive normal float or 0 here (|x| <= 1)
-    const float lnx = x > 0.0f ? logf_tab(x) : logf_acc(x);
+    const float lnx = x > 0.0f ? logf_tab(x) : -__builtin_inff();
     if(isnan(lnx))
         return 1.0f;
     if(isinf(lnx))

@@ -1202,13 +1202,20 @@ __host__ __device__ __forceinline__ double mrg_f64(unsigned int v)
            * (MRG == 32 ? ::rocrand_amd_engine::MRG32_NORM_DOUBLE : ::rocrand_amd_engine::MRG31_NORM_DOUBLE);
 }
 
-// rocrand_normal.h:51-66 (angle 2^-32 2 pi + y 2^-32 2 pi)
+// rocrand_normal.h:51-66 (angle 2^-32 2 pi + y 2^-32 2 pi).  LN selects, on the device, the
+// variate the log-normals take: the reference's own float steps (logf, sqrtf, sinf/cosf,
+// each rounded), which exp does not magnify beyond the log-normal bar; the normals take
+// the pair evaluated in double and rounded once.
+template<bool LN = false>
 __host__ __device__ __forceinline__ float2 box_muller(unsigned int x, unsigned int y)
 {
     ROCRAND_AMD_FP_EXACT
 #if defined(__HIP_DEVICE_COMPILE__)
     float r0, r1;
-    ::rocrand_amd_engine::box_muller(x, y, r0, r1);
+    if constexpr(LN)
+        ::rocrand_amd_engine::box_muller_stepwise(x, y, r0, r1);
+    else
+        ::rocrand_amd_engine::box_muller(x, y, r0, r1);
     return float2{r0, r1};
 #else
     const float u = u_f32(x);
@@ -1217,9 +1224,10 @@ __host__ __device__ __forceinline__ float2 box_muller(unsigned int x, unsigned i
     return float2{sinf(v) * s, cosf(v) * s};
 #endif
 }
+template<bool LN = false>
 __host__ __device__ __forceinline__ float2 box_muller(unsigned long long v)
 {
-    return box_muller((unsigned int)v, (unsigned int)(v >> 32));
+    return box_muller<LN>((unsigned int)v, (unsigned int)(v >> 32));
 }
 // rocrand_normal.h:76-96
 __host__ __device__ __forceinline__ double2 box_muller_double(unsigned int a, unsigned int b, unsigned int c,
@@ -1246,13 +1254,16 @@ __host__ __device__ __forceinline__ double2 box_muller_double(ulonglong2 v)
                              (unsigned int)(v.y >> 32));
 }
 // rocrand_normal.h:138-172
-template<int MRG>
+template<int MRG, bool LN = false>
 __host__ __device__ __forceinline__ float2 mrg_box_muller(unsigned int x, unsigned int y)
 {
     ROCRAND_AMD_FP_EXACT
 #if defined(__HIP_DEVICE_COMPILE__)
     float r0, r1;
-    ::rocrand_amd_engine::mrg_box_muller<MRG>(x, y, r0, r1);
+    if constexpr(LN)
+        ::rocrand_amd_engine::mrg_box_muller_stepwise<MRG>(x, y, r0, r1);
+    else
+        ::rocrand_amd_engine::mrg_box_muller<MRG>(x, y, r0, r1);
     return float2{r0, r1};
 #else
     const float u = mrg_f32<MRG>(x);
@@ -1400,24 +1411,26 @@ namespace rocrand_device
 namespace amd_detail
 {
 // Box-Muller draws: two words (the MRGs use their raw values in [1, m1]).
+template<bool LN = false>
 __host__ __device__ __forceinline__ float2 bm2(rocrand_state_philox4x32_10* s)
 {
     const unsigned int a = rocrand(s);
     const unsigned int b = rocrand(s);
-    return box_muller(a, b);
+    return box_muller<LN>(a, b);
 }
+template<bool LN = false>
 __host__ __device__ __forceinline__ float2 bm2(rocrand_state_xorwow* s)
 {
     const unsigned int a = rocrand(s);
     const unsigned int b = rocrand(s);
-    return box_muller(a, b);
+    return box_muller<LN>(a, b);
 }
-template<int MRG>
+template<bool LN = false, int MRG>
 __host__ __device__ __forceinline__ float2 bm2(mrg_engine<MRG>* s)
 {
     const unsigned int a = s->next();
     const unsigned int b = s->next();
-    return mrg_box_muller<MRG>(a, b);
+    return mrg_box_muller<MRG, LN>(a, b);
 }
 template<int MRG>
 __host__ __device__ __forceinline__ double2 bmd2(mrg_engine<MRG>* s)
@@ -1679,6 +1692,45 @@ namespace rocrand_device
 {
 namespace amd_detail
 {
+// The Box-Muller pairs of rocrand_normal2 with the log-normal variate (box_muller<true>),
+// and rocrand_normal's single value sharing the state's Box-Muller cache.
+__host__ __device__ __forceinline__ float2 normal2_ln(rocrand_state_philox4x32_10* s) { return bm2<true>(s); }
+__host__ __device__ __forceinline__ float2 normal2_ln(rocrand_state_xorwow* s) { return bm2<true>(s); }
+__host__ __device__ __forceinline__ float2 normal2_ln(rocrand_state_mrg32k3a* s) { return bm2<true>(s); }
+__host__ __device__ __forceinline__ float2 normal2_ln(rocrand_state_mrg31k3p* s) { return bm2<true>(s); }
+__host__ __device__ __forceinline__ float2 normal2_ln(rocrand_state_lfsr113* s)
+{
+    const unsigned int a = rocrand(s);
+    const unsigned int b = rocrand(s);
+    return box_muller<true>(a, b);
+}
+__host__ __device__ __forceinline__ float2 normal2_ln(rocrand_state_threefry2x32_20* s)
+{
+    const uint2 v = rocrand2(s);
+    return box_muller<true>(v.x, v.y);
+}
+__host__ __device__ __forceinline__ float2 normal2_ln(rocrand_state_threefry4x32_20* s)
+{
+    const unsigned int a = rocrand(s);
+    const unsigned int b = rocrand(s);
+    return box_muller<true>(a, b);
+}
+__host__ __device__ __forceinline__ float2 normal2_ln(rocrand_state_threefry4x64_20* s)
+{
+    const unsigned long long a = rocrand(s);
+    const unsigned long long b = rocrand(s);
+    return box_muller<true>((unsigned int)a, (unsigned int)b);
+}
+#define ROCRAND_AMD_NORMAL_LN(STATE)                                                                         \
+    __host__ __device__ __forceinline__ float normal_ln(STATE* state)                                        \
+    {                                                                                                        \
+        ROCRAND_AMD_BM_FLOAT(state, normal2_ln(state))                                                       \
+    }
+ROCRAND_AMD_NORMAL_LN(rocrand_state_philox4x32_10)
+ROCRAND_AMD_NORMAL_LN(rocrand_state_mrg32k3a)
+ROCRAND_AMD_NORMAL_LN(rocrand_state_mrg31k3p)
+ROCRAND_AMD_NORMAL_LN(rocrand_state_xorwow)
+#undef ROCRAND_AMD_NORMAL_LN
 __host__ __device__ __forceinline__ float ln_f(float mean, float stddev, float r)
 {
     ROCRAND_AMD_FP_EXACT
@@ -1694,7 +1746,11 @@ __host__ __device__ __forceinline__ float ln_fd(double mean, double stddev, floa
 __host__ __device__ __forceinline__ double ln_d(double mean, double stddev, double r)
 {
     ROCRAND_AMD_FP_EXACT
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ::rocrand_amd_engine::exp_cr(mean + stddev * r);
+#else
     return exp(mean + stddev * r);
+#endif
 }
 } // namespace amd_detail
 } // namespace rocrand_device
@@ -1702,7 +1758,7 @@ __host__ __device__ __forceinline__ double ln_d(double mean, double stddev, doub
 #define ROCRAND_AMD_LN_FROM_NORMAL(STATE)                                                                    \
     __host__ __device__ __forceinline__ float2 rocrand_log_normal2(STATE* state, float mean, float stddev)   \
     {                                                                                                        \
-        const float2 r = rocrand_normal2(state);                                                             \
+        const float2 r = rocrand_device::amd_detail::normal2_ln(state);                                      \
         return float2{rocrand_device::amd_detail::ln_f(mean, stddev, r.x),                                   \
                       rocrand_device::amd_detail::ln_f(mean, stddev, r.y)};                                  \
     }                                                                                                        \
@@ -1718,7 +1774,7 @@ __host__ __device__ __forceinline__ double ln_d(double mean, double stddev, doub
     ROCRAND_AMD_LN_FROM_NORMAL(STATE)                                                                        \
     __host__ __device__ __forceinline__ float rocrand_log_normal(STATE* state, float mean, float stddev)     \
     {                                                                                                        \
-        return rocrand_device::amd_detail::ln_f(mean, stddev, rocrand_normal(state));                        \
+        return rocrand_device::amd_detail::ln_f(mean, stddev, rocrand_device::amd_detail::normal_ln(state)); \
     }                                                                                                        \
     __host__ __device__ __forceinline__ double rocrand_log_normal_double(STATE* state, double mean,          \
                                                                          double stddev)                      \
@@ -1769,7 +1825,7 @@ __host__ __device__ __forceinline__ float2 rocrand_log_normal2(rocrand_state_thr
     using namespace rocrand_device::amd_detail;
     const unsigned long long a = rocrand(state);
     const unsigned long long b = rocrand(state);
-    const float2             r = box_muller((unsigned int)a, (unsigned int)b);
+    const float2             r = box_muller<true>((unsigned int)a, (unsigned int)b);
     return float2{ln_f(mean, stddev, r.x), ln_f(mean, stddev, r.y)};
 }
 __host__ __device__ __forceinline__ double rocrand_log_normal_double(rocrand_state_threefry2x64_20* state,
@@ -1803,8 +1859,11 @@ __host__ __device__ __forceinline__ float4 rocrand_log_normal4(rocrand_state_phi
                                                                float stddev)
 {
     using namespace rocrand_device::amd_detail;
-    const float4 r = rocrand_normal4(state);
-    return float4{ln_f(mean, stddev, r.x), ln_f(mean, stddev, r.y), ln_f(mean, stddev, r.z), ln_f(mean, stddev, r.w)};
+    const uint4  v  = rocrand4(state);
+    const float2 r1 = box_muller<true>(v.x, v.y);
+    const float2 r2 = box_muller<true>(v.z, v.w);
+    return float4{ln_f(mean, stddev, r1.x), ln_f(mean, stddev, r1.y), ln_f(mean, stddev, r2.x),
+                  ln_f(mean, stddev, r2.y)};
 }
 __host__ __device__ __forceinline__ double4 rocrand_log_normal_double4(rocrand_state_philox4x32_10* state,
                                                                        double mean, double stddev)
@@ -2032,7 +2091,9 @@ __device__ __forceinline__ float rocrand_log_normal(rocrand_state_mtgp32* state,
 __device__ __forceinline__ float2 rocrand_log_normal2(rocrand_state_mtgp32* state, float mean, float stddev)
 {
     using namespace rocrand_device::amd_detail;
-    const float2 r = rocrand_normal2(state);
+    const unsigned int a = rocrand(state);
+    const unsigned int b = rocrand(state);
+    const float2       r = box_muller<true>(a, b);
     return float2{ln_f(mean, stddev, r.x), ln_f(mean, stddev, r.y)};
 }
 __device__ __forceinline__ double rocrand_log_normal_double(rocrand_state_mtgp32* state, double mean, double stddev)

@@ -310,6 +310,8 @@ rocrand_status rocrand_generator_base_type::make_params(out_kind k, const gen_ar
     p.sd = args.b;
     p.mh = args.ha;
     p.sh = args.hb;
+    // the standard normal's affine step is the identity (mean +0: -0 + x would differ)
+    p.unit = k == out_kind::normal_f32 && __builtin_bit_cast(uint32_t, p.mf) == 0u && p.sf == 1.0f;
     if(k == out_kind::poisson)
         return m_poisson.get(args.a, stream, p);
     return ROCRAND_STATUS_SUCCESS;
@@ -664,10 +666,10 @@ public:
     {
         if(m_initialized)
             return ROCRAND_STATUS_SUCCESS;
-        const int words = inter_state_words(m_kind);
         if(!m_states)
         {
-            if(hipMalloc(&m_states, (size_t)words * INTERLEAVED_ENGINES * sizeof(uint32_t)) != hipSuccess)
+            // two state buffers: a call reads one and leaves the advanced states in the other
+            if(hipMalloc(&m_states, 2 * state_words() * sizeof(uint32_t)) != hipSuccess)
             {
                 m_states = nullptr;
                 return ROCRAND_STATUS_ALLOCATION_FAILED;
@@ -676,7 +678,7 @@ public:
         m_start = (uint32_t)(offset % INTERLEAVED_ENGINES);
         uint32_t seeded[6];
         seed_state(seeded);
-        if(launch_interleaved_init(m_kind, m_states, seeded, m_start, offset / INTERLEAVED_ENGINES, stream)
+        if(launch_interleaved_init(m_kind, cur_states(), seeded, m_start, offset / INTERLEAVED_ENGINES, stream)
            != hipSuccess)
             return ROCRAND_STATUS_LAUNCH_FAILURE;
         m_initialized = true;
@@ -739,12 +741,14 @@ protected:
         c.head   = (uint32_t)std::min<size_t>(n, c.mis);
         c.tail   = (uint32_t)((n - c.head) % s.out_w);
         c.vec_n  = (n - c.head) / s.out_w;
-        c.states = m_states;
-        c.start  = m_start;
+        c.states     = cur_states();
+        c.states_out = m_states + (m_cur ^ 1u) * state_words();
+        c.start      = m_start;
         c.data   = data;
         c.n      = n;
         if(launch_interleaved(m_kind, k, c, stream) != hipSuccess)
             return ROCRAND_STATUS_LAUNCH_FAILURE;
+        m_cur ^= 1u;
         // mrg.hpp:385-389
         m_start = (uint32_t)((m_start + (n + s.out_w - 1) / s.out_w) % INTERLEAVED_ENGINES);
         return ROCRAND_STATUS_SUCCESS;
@@ -755,8 +759,12 @@ private:
     unsigned long long m_seed        = 0;
     uint4              m_seed4       = {2, 8, 16, 128};
     bool               m_initialized = false;
-    uint32_t*          m_states      = nullptr;
+    uint32_t*          m_states      = nullptr; // 2 * state_words() words
+    uint32_t           m_cur         = 0;       // which half holds the current states
     uint32_t           m_start       = 0;
+
+    size_t    state_words() const { return (size_t)inter_state_words(m_kind) * INTERLEAVED_ENGINES; }
+    uint32_t* cur_states() const { return m_states + m_cur * state_words(); }
 };
 
 } // namespace

@@ -106,7 +106,9 @@ hipError_t dispatch_kind(out_kind o, bool one_word, const counter_call& c, hipSt
         case out_kind::uniform_f32: return launch_one<K, dist<F, out_kind::uniform_f32>>(c, s);
         case out_kind::uniform_f64: return launch_one<K, dist<F, out_kind::uniform_f64>>(c, s);
         case out_kind::uniform_f16: return launch_one<K, dist<F, out_kind::uniform_f16>>(c, s);
-        case out_kind::normal_f32: return launch_one<K, dist<F, out_kind::normal_f32>>(c, s);
+        case out_kind::normal_f32:
+            return c.params.unit ? launch_one<K, dist<F, out_kind::normal_f32, true>>(c, s)
+                                 : launch_one<K, dist<F, out_kind::normal_f32>>(c, s);
         case out_kind::lognormal_f32: return launch_one<K, dist<F, out_kind::lognormal_f32>>(c, s);
         case out_kind::normal_f16: return launch_one<K, dist<F, out_kind::normal_f16>>(c, s);
         case out_kind::lognormal_f16: return launch_one<K, dist<F, out_kind::lognormal_f16>>(c, s);

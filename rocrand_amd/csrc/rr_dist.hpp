@@ -88,6 +88,7 @@ struct dist_params
     alias_table pt;
     int         huge;
     double      lambda, sqrt_lambda;
+    int         unit; // mf is +0 and sf is 1 (the standard normal)
 };
 
 // Family of the engine feeding the transform: generic 32-bit words, generic
@@ -100,8 +101,29 @@ enum class word_family : int
     mrg31
 };
 
-template<word_family F, out_kind K, bool NormalDoubleOneWord = false>
+// B selects a variant: for normal_f64 / lognormal_f64 of ThreeFry2x32 the one-word
+// inverse-error-function transform; for normal_f32 the standard normal (mean +0, stddev 1),
+// whose affine step is the identity but for the sign of a zero (mean + 1 * x is +0 there).
+template<word_family F, out_kind K, bool B = false>
 struct dist;
+
+// mean + x * stddev in float, rounded twice as the reference does (normal.hpp), or the
+// pair as it is for the standard normal (the Box-Muller pair then rounded x + 0).
+template<bool UNIT>
+__device__ __forceinline__ void rr_affine(float r0, float r1, float* out, const dist_params& p)
+{
+    if constexpr(UNIT)
+    {
+        (void)p;
+        out[0] = r0;
+        out[1] = r1;
+    }
+    else
+    {
+        out[0] = __fadd_rn(p.mf, __fmul_rn(r0, p.sf));
+        out[1] = __fadd_rn(p.mf, __fmul_rn(r1, p.sf));
+    }
+}
 
 // The output kind of a transform type.
 template<class D>
@@ -150,9 +172,8 @@ RR_DIST(word_family::w32, out_kind::uniform_f16, 1, 2, __half)
 RR_DIST(word_family::w32, out_kind::normal_f32, 2, 2, float)
 {
     float r0, r1;
-    box_muller((uint32_t)in[0], (uint32_t)in[1], r0, r1);
-    out[0] = __fadd_rn(p.mf, __fmul_rn(r0, p.sf));
-    out[1] = __fadd_rn(p.mf, __fmul_rn(r1, p.sf));
+    box_muller<B>((uint32_t)in[0], (uint32_t)in[1], r0, r1);
+    rr_affine<B>(r0, r1, out, p);
 } };
 RR_DIST(word_family::w32, out_kind::lognormal_f32, 2, 2, float)
 {
@@ -218,8 +239,8 @@ struct dist<word_family::w32, out_kind::lognormal_f64, false>
     {
         double r0, r1;
         box_muller_double((uint32_t)in[0], (uint32_t)in[1], (uint32_t)in[2], (uint32_t)in[3], r0, r1);
-        out[0] = exp(__dadd_rn(p.md, __dmul_rn(r0, p.sd)));
-        out[1] = exp(__dadd_rn(p.md, __dmul_rn(r1, p.sd)));
+        out[0] = exp_cr(__dadd_rn(p.md, __dmul_rn(r0, p.sd)));
+        out[1] = exp_cr(__dadd_rn(p.md, __dmul_rn(r1, p.sd)));
     }
 };
 template<>
@@ -243,7 +264,7 @@ struct dist<word_family::w32, out_kind::lognormal_f64, true>
     __device__ __forceinline__ static void apply(const W* in, double* out, const dist_params& p)
     {
         const double v = normal_double_1(__dadd_rn(R_2POW32_INV_DBL, __dmul_rn((double)(uint32_t)in[0], R_2POW32_INV_DBL)));
-        out[0] = exp(__dadd_rn(p.md, __dmul_rn(v, p.sd)));
+        out[0] = exp_cr(__dadd_rn(p.md, __dmul_rn(v, p.sd)));
     }
 };
 
@@ -282,9 +303,8 @@ RR_DIST(word_family::w64, out_kind::normal_f32, 1, 2, float)
 {
     float r0, r1;
     const uint64_t v = (uint64_t)in[0];
-    box_muller((uint32_t)v, (uint32_t)(v >> 32), r0, r1);
-    out[0] = __fadd_rn(p.mf, __fmul_rn(r0, p.sf));
-    out[1] = __fadd_rn(p.mf, __fmul_rn(r1, p.sf));
+    box_muller<B>((uint32_t)v, (uint32_t)(v >> 32), r0, r1);
+    rr_affine<B>(r0, r1, out, p);
 } };
 RR_DIST(word_family::w64, out_kind::lognormal_f32, 1, 2, float)
 {
@@ -307,8 +327,8 @@ RR_DIST(word_family::w64, out_kind::lognormal_f64, 2, 2, double)
     double r0, r1;
     const uint64_t a = (uint64_t)in[0], b = (uint64_t)in[1];
     box_muller_double((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32), r0, r1);
-    out[0] = exp(__dadd_rn(p.md, __dmul_rn(r0, p.sd)));
-    out[1] = exp(__dadd_rn(p.md, __dmul_rn(r1, p.sd)));
+    out[0] = exp_cr(__dadd_rn(p.md, __dmul_rn(r0, p.sd)));
+    out[1] = exp_cr(__dadd_rn(p.md, __dmul_rn(r1, p.sd)));
 } };
 RR_DIST(word_family::w64, out_kind::normal_f16, 1, 2, __half)
 {
@@ -366,9 +386,8 @@ RR_DIST(word_family::w64, out_kind::poisson, 1, 1, uint32_t)
     RR_DIST(F, out_kind::normal_f32, 2, 2, float)                                                \
     {                                                                                            \
         float r0, r1;                                                                            \
-        mrg_box_muller<M>((uint32_t)in[0], (uint32_t)in[1], r0, r1);                             \
-        out[0] = __fadd_rn(p.mf, __fmul_rn(r0, p.sf));                                           \
-        out[1] = __fadd_rn(p.mf, __fmul_rn(r1, p.sf));                                           \
+        mrg_box_muller<M, B>((uint32_t)in[0], (uint32_t)in[1], r0, r1);                          \
+        rr_affine<B>(r0, r1, out, p);                                                            \
     } };                                                                                         \
     RR_DIST(F, out_kind::lognormal_f32, 2, 2, float)                                             \
     {                                                                                            \
@@ -388,8 +407,8 @@ RR_DIST(word_family::w64, out_kind::poisson, 1, 1, uint32_t)
     {                                                                                            \
         double r0, r1;                                                                           \
         mrg_box_muller_double<M>((uint32_t)in[0], (uint32_t)in[1], r0, r1);                      \
-        out[0] = exp(__dadd_rn(p.md, __dmul_rn(r0, p.sd)));                                      \
-        out[1] = exp(__dadd_rn(p.md, __dmul_rn(r1, p.sd)));                                      \
+        out[0] = exp_cr(__dadd_rn(p.md, __dmul_rn(r0, p.sd)));                                      \
+        out[1] = exp_cr(__dadd_rn(p.md, __dmul_rn(r1, p.sd)));                                      \
     } };                                                                                         \
     RR_DIST(F, out_kind::normal_f16, 1, 2, __half)                                               \
     {                                                                                            \

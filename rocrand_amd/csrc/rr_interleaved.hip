@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256) void generate_kernel(interleaved_call c)
                     data[c.n - c.tail + j] = o[j];
         }
     }
-    g.store(c.states, eid);
+    g.store(c.states_out, eid);
 }
 
 // Long calls: each engine's vec-steps are split into J chunks of C, one lane per
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(256) void generate_chunked_kernel(interleaved_call 
                     data[c.n - c.tail + q] = o[q];
         }
     }
-    eng.store(c.states, eid);
+    eng.store(c.states_out, eid);
 }
 
 template<inter_kind K, class D>
@@ -322,7 +322,9 @@ hipError_t dispatch(out_kind o, const interleaved_call& c, hipStream_t s)
         case out_kind::uniform_f32: return launch_gen<K, dist<F, out_kind::uniform_f32>>(c, s);
         case out_kind::uniform_f64: return launch_gen<K, dist<F, out_kind::uniform_f64>>(c, s);
         case out_kind::uniform_f16: return launch_gen<K, dist<F, out_kind::uniform_f16>>(c, s);
-        case out_kind::normal_f32: return launch_gen<K, dist<F, out_kind::normal_f32>>(c, s);
+        case out_kind::normal_f32:
+            return c.params.unit ? launch_gen<K, dist<F, out_kind::normal_f32, true>>(c, s)
+                                 : launch_gen<K, dist<F, out_kind::normal_f32>>(c, s);
         case out_kind::normal_f64: return launch_gen<K, dist<F, out_kind::normal_f64>>(c, s);
         case out_kind::normal_f16: return launch_gen<K, dist<F, out_kind::normal_f16>>(c, s);
         case out_kind::lognormal_f32: return launch_gen<K, dist<F, out_kind::lognormal_f32>>(c, s);

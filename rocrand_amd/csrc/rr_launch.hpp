@@ -40,7 +40,9 @@ hipError_t launch_counter(ctr_kind k, out_kind o, bool normal_one_word, const co
 // One interleaved generate call.
 struct interleaved_call
 {
-    uint32_t*   states;   // structure-of-arrays engine states, word j of engine e at [j*S + e]
+    uint32_t*   states;     // structure-of-arrays engine states, word j of engine e at [j*S + e]
+    uint32_t*   states_out; // where the call leaves the advanced states (same layout; a second
+                            // buffer, so no block of a launch reads a state another block writes)
     uint32_t    start;    // start engine
     void*       data;
     size_t      n, mis;

@@ -92,7 +92,7 @@ __device__ __forceinline__ typename sob_out<K>::type sob_apply(V x, const dist_p
     else if constexpr(K == out_kind::lognormal_f32)
         return expf_acc(__fadd_rn(p.mf, __fmul_rn(p.sf, sob_normal_f32(x))));
     else if constexpr(K == out_kind::lognormal_f64)
-        return exp(__dadd_rn(p.md, __dmul_rn(p.sd, sob_normal_f64(x))));
+        return exp_cr(__dadd_rn(p.md, __dmul_rn(p.sd, sob_normal_f64(x))));
     else if constexpr(K == out_kind::lognormal_f16)
         return __float2half(
             expf_acc(__fadd_rn(__half2float(p.mh), __fmul_rn(__half2float(p.sh), sob_normal_f32(x)))));

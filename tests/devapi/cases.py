@@ -81,6 +81,16 @@ def components(engine, fn):
     return 1
 
 
+def _sobol32_safe(engine, off):
+    if engine in ("sobol32", "scrambled_sobol32"):
+        # keep the 32-bit point index below 2^32 - 256: at index 2^32 - 1 the
+        # reference's host branch reads past the direction vectors
+        # (rocrand_sobol32.h rightmost_zero_bit returns 32 for an all-ones index)
+        off = off.copy()
+        off[(off & np.uint64(0xFFFFFFFF)) >= np.uint64(0xFFFFFF00)] ^= np.uint64(0x100)
+    return off
+
+
 def _bind(lib, name):
     f = getattr(lib, name)
     f.restype = ctypes.c_int
@@ -103,12 +113,7 @@ class Driver:
         n = len(sub)
         sub = np.ascontiguousarray(sub, dtype=np.uint64)
         off = np.ascontiguousarray(off, dtype=np.uint64)
-        if engine in ("sobol32", "scrambled_sobol32"):
-            # keep the 32-bit point index below 2^32 - 256: at index 2^32 - 1 the
-            # reference's host branch reads past the direction vectors
-            # (rocrand_sobol32.h rightmost_zero_bit returns 32 for an all-ones index)
-            off = off.copy()
-            off[(off & np.uint64(0xFFFFFFFF)) >= np.uint64(0xFFFFFF00)] ^= np.uint64(0x100)
+        off = _sobol32_safe(engine, off)
         prob, alias, cdf, doff = table if table is not None else discrete_table()
         prob = np.ascontiguousarray(prob, dtype=np.float64)
         alias = np.ascontiguousarray(alias, dtype=np.uint32)
@@ -120,6 +125,35 @@ class Driver:
         rc = self.fn(E[engine], fn, seed, sub.ctypes.data, off.ctypes.data, n, calls, a, b, len(prob), doff,
                      prob.ctypes.data, alias.ctypes.data, cdf.ctypes.data, v32.ctypes.data, v64.ctypes.data,
                      len(v32) // 32, out.ctypes.data)
+        if rc == -1:
+            return None
+        assert rc == 0, f"driver returned {rc}"
+        return out.reshape(n, calls, 4)
+
+
+class HandleDriver:
+    """The GPU kernels over a device-resident discrete distribution handle (DISCRETE and
+    DISCRETE4 only)."""
+
+    def __init__(self):
+        self.lib = ctypes.CDLL(GPU_LIB)
+        f = self.lib.devapi_gpu_run_handle
+        f.restype = ctypes.c_int
+        f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                      ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint,
+                      ctypes.c_void_p]
+        self.fn = f
+
+    def run(self, engine, fn, sub, off, calls, handle, seed=SEED, vectors=None):
+        n = len(sub)
+        sub = np.ascontiguousarray(sub, dtype=np.uint64)
+        off = _sobol32_safe(engine, np.ascontiguousarray(off, dtype=np.uint64))
+        v32, v64 = vectors if vectors is not None else sobol_vectors()
+        v32 = np.ascontiguousarray(v32, dtype=np.uint32)
+        v64 = np.ascontiguousarray(v64, dtype=np.uint64)
+        out = np.zeros(n * calls * 4, dtype=np.uint64)
+        rc = self.fn(E[engine], fn, seed, sub.ctypes.data, off.ctypes.data, n, calls, handle, v32.ctypes.data,
+                     v64.ctypes.data, len(v32) // 32, out.ctypes.data)
         if rc == -1:
             return None
         assert rc == 0, f"driver returned {rc}"

@@ -118,6 +118,69 @@ extern "C" int devapi_gpu_run(int engine, int fn, unsigned long long seed, const
     return 0;
 }
 
+// The same kernels over a discrete distribution that already lives on the device (one
+// made by rocrand_create_discrete_distribution / rocrand_create_poisson_distribution).
+extern "C" int devapi_gpu_run_handle(int engine, int fn, unsigned long long seed, const unsigned long long* sub,
+                                     const unsigned long long* off, int n, int calls,
+                                     rocrand_discrete_distribution dist, const unsigned int* v32,
+                                     const unsigned long long* v64, unsigned int dims, unsigned long long* out)
+{
+    const bool sobol = engine >= devapi_test::SOBOL32;
+    if(n <= 0 || calls <= 0 || (size_t)n * calls > (1u << 26) || dist == nullptr
+       || (fn != devapi_test::DISCRETE && fn != devapi_test::DISCRETE4) || (sobol && (dims == 0 || !v32 || !v64)))
+        return -2;
+    bool known = false;
+    devapi_test::with_engine(engine,
+                             [&](auto* tag)
+                             {
+                                 using S = std::remove_pointer_t<decltype(tag)>;
+                                 known   = devapi_test::supported<S>(fn);
+                             });
+    if(!known)
+        return -1;
+    device_buffers     buf;
+    const size_t       out_bytes = (size_t)n * calls * 4 * sizeof(uint64_t);
+    const unsigned int nd        = sobol ? dims : 1;
+    if(hipMalloc(&buf.p[0], n * sizeof(unsigned long long)) != hipSuccess
+       || hipMalloc(&buf.p[1], n * sizeof(unsigned long long)) != hipSuccess
+       || hipMalloc(&buf.p[2], out_bytes) != hipSuccess
+       || hipMalloc(&buf.p[7], (size_t)nd * 32 * sizeof(unsigned int)) != hipSuccess
+       || hipMalloc(&buf.p[8], (size_t)nd * 64 * sizeof(unsigned long long)) != hipSuccess)
+        return -3;
+    if(hipMemcpy(buf.p[0], sub, n * sizeof(unsigned long long), hipMemcpyHostToDevice) != hipSuccess
+       || hipMemcpy(buf.p[1], off, n * sizeof(unsigned long long), hipMemcpyHostToDevice) != hipSuccess
+       || hipMemset(buf.p[2], 0, out_bytes) != hipSuccess)
+        return -3;
+    if(sobol
+       && (hipMemcpy(buf.p[7], v32, (size_t)nd * 32 * sizeof(unsigned int), hipMemcpyHostToDevice) != hipSuccess
+           || hipMemcpy(buf.p[8], v64, (size_t)nd * 64 * sizeof(unsigned long long), hipMemcpyHostToDevice)
+                  != hipSuccess))
+        return -3;
+    const devapi_test::sobol_tables tables = {(const unsigned int*)buf.p[7], (const unsigned long long*)buf.p[8], nd};
+    const unsigned int              blocks = (unsigned int)((n + BLOCK - 1) / BLOCK);
+    devapi_test::with_engine(
+        engine,
+        [&](auto* tag)
+        {
+            using S = std::remove_pointer_t<decltype(tag)>;
+            devapi_test::with_fn(fn,
+                                 [&](auto fc)
+                                 {
+                                     constexpr int FN = decltype(fc)::value;
+                                     if constexpr(devapi_test::supported<S>(FN))
+                                         hipLaunchKernelGGL((devapi_kernel<S, FN>), dim3(blocks), dim3(BLOCK), 0, 0,
+                                                            seed, (const unsigned long long*)buf.p[0],
+                                                            (const unsigned long long*)buf.p[1], n, calls, 0.0, 0.0,
+                                                            dist, tables, (uint64_t*)buf.p[2]);
+                                 });
+        });
+    if(hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        return -4;
+    if(hipMemcpy(out, buf.p[2], out_bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        return -3;
+    return 0;
+}
+
 // ---------------------------------------------------------------- MTGP32
 namespace
 {

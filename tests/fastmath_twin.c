@@ -10,13 +10,15 @@
  *   log  [2^-32, 1]   sin, cos  (0, 7]   exp  (-105, 90)
  *
  * usage: fastmath_twin bm N [mrg]
- * The Box-Muller pair of box_muller_uv (one double expression rounded once) over N
+ * The Box-Muller pair of box_muller_dd (one double expression rounded once) over N
  * pseudo-random word pairs (32-bit words, or MRG32k3a values in [1, m1] with "mrg"):
  *   bm N  COUNT_BEYOND_2ULP_OF_HOST  MAX_ULP_TO_HOST  MAX_ERR_MILLIULP  HOST_MAX_ERR_MILLIULP
  * where the host is the reference's float pipeline over this libm (sqrtf(-2 logf(u)),
  * sinf(v) s, cosf(v) s) and the error is against the long double evaluation.  The
- * device's float reciprocal square root is emulated by a correctly rounded one: one
- * Newton step leaves the radius insensitive to the seed's last bits.
+ * device's double reciprocal square root (within 2^-22 relative) is emulated by a
+ * correctly rounded one: one Newton step leaves the radius insensitive to the seed's
+ * last bits.  The angle's table entry comes from the word as on the device (q =
+ * round(y 2^-24) for 32-bit words, round(y 2^-24) too for MRG32k3a values).
  */
 #include <math.h>
 #include <stdint.h>
@@ -80,28 +82,25 @@ static float expf_acc(float xf)
     return xf != xf ? xf : y;
 }
 
-static void box_muller_uv(float u, float v, float* r0, float* r1)
+/* box_muller_dd: the pair in double, rounded once; qoff = 16 q (the angle table entry) */
+static void box_muller_dd(float u, float v, uint32_t qoff, float* r0, float* r1)
 {
-    const uint32_t ix = fbits(u), tmp = ix - LOGTAB_OFF;
-    const uint32_t i = (tmp >> (23 - LOGTAB_BITS)) & ((1u << LOGTAB_BITS) - 1u);
-    const int k = (int32_t)tmp >> 23;
-    const float z = bitsf(ix - (tmp & 0xff800000u));
-    const double r = fma((double)z, LOGTAB[i][0], -1.0), r2 = r * r;
-    double q = fma(r, -0.25, 1.0 / 3);
-    q = fma(q, r, -0.5);
-    const double lnu = fma((double)k, LN2, LOGTAB[i][1]) + fma(q, r2, r);
-    const double x = -2.0 * lnu;
-    const double y = (double)(1.0f / sqrtf(fmaxf((float)x, 0x1p-126f)));
+    const uint32_t loff = (fbits(u) >> 13) & 0x1fff0u;
+    const double* lt = BMLOGTAB[loff / 16 - BM_EMIN * 64];
+    const double r = fma((double)u, lt[0], -1.0);
+    double p = fma(r, 0.5, -2.0 / 3);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, -2.0);
+    const double x = fma(p, r, lt[1]);
+    const double y = 1.0 / sqrt(x > 0x1p-126 ? x : 0x1p-126);
     const double t = x * y, h = fma(-t, y, 1.0), s = fma(t * h, 0.5, t);
-    const double vd = v, qv = rint(vd * SC_INV_STEP);
-    const int kv = (int)qv & ((1 << SCTAB_BITS) - 1);
-    double a = fma(-qv, SC_STEP_HI, vd);
-    a = fma(-qv, SC_STEP_LO, a);
+    const double* st = BMSCTAB[qoff / 16];
+    const double a = (double)fmaf(-(float)qoff, BM_ANGLE_STEP * 0.0625f, v);
     const double a2 = a * a;
-    const double sa = fma(a * a2, fma(a2, 1.0 / 120, -1.0 / 6), a);
+    const double sa = a * fma(a2, -1.0 / 6, 1.0);
     const double ca = fma(a2, fma(a2, 1.0 / 24, -0.5), 1.0);
-    const double sn = fma(SCTAB[kv][0], ca, SCTAB[kv][1] * sa);
-    const double cs = fma(SCTAB[kv][1], ca, -(SCTAB[kv][0] * sa));
+    const double sn = fma(st[0], ca, st[1] * sa);
+    const double cs = fma(st[1], ca, -(st[0] * sa));
     *r0 = (float)(sn * s);
     *r1 = (float)(cs * s);
 }
@@ -152,7 +151,7 @@ static int box_muller_sample(long n, int mrg)
             v = 1.46291807e-09f + (float)w[1] * 1.46291807e-09f;
         }
         float d0, d1;
-        box_muller_uv(u, v, &d0, &d1);
+        box_muller_dd(u, v, ((w[1] >> 20) + 8u) & ~15u, &d0, &d1);
         const float hs = sqrtf(-2.0f * logf(u));
         const float h[2] = {sinf(v) * hs, cosf(v) * hs};
         const long double S = sqrtl(-2.0L * logl((long double)u));

@@ -24,11 +24,19 @@ def twin_header(text):
     logs = re.findall(r"\{(0x[^,]+),\s*([^}]+)\}", log_body)
     exps = re.findall(r"(0x[0-9a-fA-Fp.+-]+),", exp_body)
     scs = re.findall(r"\{(-?0x[^,]+),\s*([^}]+)\}", sc_body)
+    bm_log_body = text[text.index("rr_bm_logtab["):text.index("};", text.index("rr_bm_logtab["))]
+    bm_sc_body = text[text.index("rr_bm_sctab["):text.index("};", text.index("rr_bm_sctab["))]
+    bm_logs = re.findall(r"\{(-?0x[^,]+),\s*([^}]+)\}", bm_log_body)
+    bm_scs = re.findall(r"\{(-?0x[^,]+),\s*([^}]+)\}", bm_sc_body)
     assert len(logs) == 64 and len(exps) == 32 and len(scs) == 64
+    assert len(bm_logs) == 33 * 64 and len(bm_scs) == 257
     out = [f"#define {k[3:]} {v.rstrip('u')}" for k, v in consts.items()]
     out.append("static const double LOGTAB[64][2] = {" + ",".join(f"{{{a},{b}}}" for a, b in logs) + "};")
     out.append("static const double EXPTAB[32] = {" + ",".join(exps) + "};")
     out.append("static const double SCTAB[64][2] = {" + ",".join(f"{{{a},{b}}}" for a, b in scs) + "};")
+    out.append("static const double BMLOGTAB[%d][2] = {" % len(bm_logs)
+               + ",".join(f"{{{a},{b}}}" for a, b in bm_logs) + "};")
+    out.append("static const double BMSCTAB[257][2] = {" + ",".join(f"{{{a},{b}}}" for a, b in bm_scs) + "};")
     return "\n".join(out) + "\n"
 
 
