@@ -76,6 +76,18 @@ ROCRANDAPI rocrand_status rocrand_batch_plan_destroy(rocrand_batch_plan plan);
 ROCRANDAPI rocrand_status rocrand_generate_batched(const rocrand_batch_stream* streams,
                                                    size_t                      count,
                                                    hipStream_t                 stream);
+/* No counterpart in the reference: the one-shot call for callers that cannot allocate
+ * device or pinned memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same streams, same outputs as rocrand_generate_batched; the
+ * descriptors go into a fixed 16 MiB array in device memory, one per GPU, held until the
+ * process ends, and no allocation routine is called.  OUT_OF_RANGE when a batch's
+ * descriptors exceed that array (about 10^6 tiles of 16 Ki values, or fewer streams
+ * with large Poisson tables).  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (issued on different streams) race; calls in order on one stream
+ * do not. */
+ROCRANDAPI rocrand_status rocrand_generate_batched_noalloc(const rocrand_batch_stream* streams,
+                                                           size_t                      count,
+                                                           hipStream_t                 stream);
 
 #if defined(__cplusplus)
 }

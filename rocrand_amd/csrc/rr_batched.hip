@@ -509,6 +509,35 @@ staging_pool& pool()
     return *p;
 }
 
+// Descriptor image of rocrand_generate_batched_noalloc: a fixed array in device memory
+// (each GPU the library runs on holds its own copy until the process ends) instead of a
+// pooled block.  Every call on a GPU reuses that GPU's copy, so calls that overlap on one
+// GPU (different streams, or the next call before the previous one's kernels have read
+// their descriptors on another stream) race; calls in order on one stream do not.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 16u << 20;
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+
+// Device address of `scratch` on the current device (in host code the array's name is a
+// host-side stand-in, not a device address), looked up once per device by the first
+// no-allocation call there.
+char* scratch_address()
+{
+    static std::mutex m;
+    static char*      addr[64] = {};
+    int               dev      = 0;
+    if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+        return nullptr;
+    std::lock_guard<std::mutex> lock(m);
+    if(addr[dev] == nullptr)
+    {
+        void* p = nullptr;
+        if(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess)
+            return nullptr;
+        addr[dev] = static_cast<char*>(p);
+    }
+    return addr[dev];
+}
+
 } // namespace
 } // namespace rr
 
@@ -888,6 +917,51 @@ rocrand_status ROCRANDAPI rocrand_generate_batched(const rocrand_batch_stream* s
         }
         pool().release_after(blk, ev);
         if(!ok)
+            return ROCRAND_STATUS_LAUNCH_FAILURE;
+    }
+    catch(const std::bad_alloc&)
+    {
+        return ROCRAND_STATUS_ALLOCATION_FAILED;
+    }
+    catch(rocrand_status st)
+    {
+        return st;
+    }
+    return ROCRAND_STATUS_SUCCESS;
+}
+
+rocrand_status ROCRANDAPI rocrand_generate_batched_noalloc(const rocrand_batch_stream* streams, size_t count,
+                                                           hipStream_t stream)
+{
+    if(streams == nullptr && count > 0)
+        return ROCRAND_STATUS_OUT_OF_RANGE;
+    try
+    {
+        batch_layout   L;
+        rocrand_status st = build(L, streams, count);
+        if(st != ROCRAND_STATUS_SUCCESS)
+            return st;
+        if(L.groups.empty())
+            return ROCRAND_STATUS_SUCCESS;
+        if(L.bytes > NOALLOC_SCRATCH_BYTES)
+            return ROCRAND_STATUS_OUT_OF_RANGE; // descriptors beyond the fixed scratch array
+        device_tables tables{};
+        char*         dev = nullptr;
+        if(get_device_tables(tables) != hipSuccess || (dev = scratch_address()) == nullptr)
+            return ROCRAND_STATUS_INTERNAL_ERROR;
+        for(auto& g : L.groups)
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mtx);
+            auto                        it = g_tuned.find(tune_key(g));
+            if(it != g_tuned.end())
+                g.blocks = it->second;
+        }
+        // Descriptors are packed in host memory and copied into the scratch array on the
+        // stream; a copy from pageable memory has read the host buffer when it returns.
+        std::vector<char> image(L.bytes);
+        L.pack(image.data(), dev);
+        if(hipMemcpyAsync(dev, image.data(), L.bytes, hipMemcpyHostToDevice, stream) != hipSuccess
+           || run_groups(L.groups, tables, stream) != hipSuccess)
             return ROCRAND_STATUS_LAUNCH_FAILURE;
     }
     catch(const std::bad_alloc&)

@@ -115,6 +115,7 @@ def load():
     L.rocrand_batch_plan_launch.argtypes = [vp]
     L.rocrand_batch_plan_destroy.argtypes = [vp]
     L.rocrand_generate_batched.argtypes = [ctypes.POINTER(BatchStream), sz, vp]
+    L.rocrand_generate_batched_noalloc.argtypes = [ctypes.POINTER(BatchStream), sz, vp]
     _lib = L
     return L
 
