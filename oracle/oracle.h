@@ -64,6 +64,7 @@ int      orc_set_seed_uint4(orc_gen* g, const unsigned int seed[4]);
 int      orc_set_offset(orc_gen* g, unsigned long long offset);
 int      orc_set_dimensions(orc_gen* g, unsigned int dimensions);
 int      orc_set_ordering(orc_gen* g, int ordering);
+int      orc_set_grid(orc_gen* g, unsigned threads, unsigned blocks);
 
 /*
  * Generate n values of `kind` into host buffer `out`.  `addr` is the address

@@ -61,6 +61,7 @@ def lib():
         L.orc_set_seed_uint4.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint)]
         L.orc_set_offset.argtypes = [ctypes.c_void_p, ctypes.c_ulonglong]
         L.orc_set_dimensions.argtypes = [ctypes.c_void_p, ctypes.c_uint]
+        L.orc_set_grid.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint]
         L.orc_generate.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
                                    ctypes.c_uint64, ctypes.c_double, ctypes.c_double]
         L.orc_engine_stream.argtypes = [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_ulonglong,
@@ -126,6 +127,7 @@ def ref():
         R.ref_engine_stream.argtypes = [ctypes.c_int, ctypes.c_ulonglong, ctypes.c_ulonglong,
                                         ctypes.c_ulonglong, ctypes.c_void_p, ctypes.c_size_t]
         R.ref_set_dimensions.argtypes = [ctypes.c_void_p, ctypes.c_uint]
+        R.ref_set_grid.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint]
         R.ref_sobol64_set_vectors.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         for name in ["ref_sobol32_vectors", "ref_scrambled_sobol32_vectors", "ref_scramble32_constants"]:
             getattr(R, name).restype = ctypes.POINTER(ctypes.c_uint)
@@ -160,11 +162,15 @@ def ref_sobol_tables():
 class _Gen:
     _prefix = "orc"
 
-    def __init__(self, rng_type, seed=None, offset=None, dims=None, seed4=None):
+    def __init__(self, rng_type, seed=None, offset=None, dims=None, seed4=None, grid=None):
         self.L = self._library()
         self.h = getattr(self.L, self._prefix + "_create")(rng_type)
         if not self.h:
             raise ValueError(f"unsupported rng type {rng_type}")
+        if grid is not None:
+            st = self.set_grid(*grid)
+            if st != 0:
+                raise ValueError(f"grid {grid}: status {st}")
         if seed is not None:
             self.set_seed(seed)
         if seed4 is not None:
@@ -195,6 +201,12 @@ class _Gen:
 
     def set_dimensions(self, d):
         return getattr(self.L, self._prefix + "_set_dimensions")(self.h, d)
+
+    def set_grid(self, threads, blocks):
+        """The generate kernels' launch grid (threads x blocks): the engine count of the
+        interleaved engines, the grid stride of the counter engines (a tuned,
+        DYNAMIC-ordering configuration)."""
+        return getattr(self.L, self._prefix + "_set_grid")(self.h, threads, blocks)
 
     def generate(self, kind, n, addr=0x10000000, a=0.0, b=1.0):
         """Return (status, array) for n values of `kind` at device address `addr`."""

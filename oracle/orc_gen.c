@@ -541,6 +541,8 @@ struct orc_gen
     ctr_engine ce;
     unsigned   start;
     void*      engines;
+    unsigned   S;          /* engines of the interleaved engines (grid threads x blocks) */
+    unsigned   ctr_stride; /* grid stride of the counter engines' generate kernel */
     struct orc_ext* ext; /* MTGP32 / MT19937 / Sobol state (orc_ext.c) */
 };
 
@@ -571,9 +573,11 @@ orc_gen* orc_create(int rng_type)
         rng_type = RNG_SOBOL32; /* ROCRAND_RNG_QUASI_DEFAULT */
     if(!is_counter(rng_type) && !is_interleaved(rng_type) && !is_ext(rng_type))
         return NULL;
-    orc_gen* g = (orc_gen*)calloc(1, sizeof(orc_gen));
-    g->type    = rng_type;
-    g->dims    = 1;
+    orc_gen* g    = (orc_gen*)calloc(1, sizeof(orc_gen));
+    g->type       = rng_type;
+    g->dims       = 1;
+    g->S          = S_ENGINES;
+    g->ctr_stride = CTR_STRIDE;
     if(rng_type == RNG_MRG32K3A || rng_type == RNG_MRG31K3P)
         g->seed = 12345ull;
     if(rng_type == RNG_LFSR113)
@@ -669,8 +673,29 @@ int orc_set_dimensions(orc_gen* g, unsigned int dims)
 
 int orc_set_ordering(orc_gen* g, int ordering)
 {
-    (void)ordering; /* every pseudo ordering maps to the default grid on gfx950 */
+    (void)ordering; /* the grid is set by orc_set_grid (the DEFAULT grid unless changed) */
     if(!g) return ST_NOT_CREATED;
+    return ST_SUCCESS;
+}
+
+/* The launch grid the reference's generate kernels would run with under a tuned
+ * (DYNAMIC-ordering) configuration (config_types.hpp:314-336): only threads x blocks
+ * matters to the values -- the engine count of the interleaved engines (mrg.hpp:284-337,
+ * get_least_common_grid_size) and the grid stride of the counter engines'
+ * kernels (philox4x32_10.hpp:106-214).  Resets the generator like set_offset. */
+int orc_set_grid(orc_gen* g, unsigned threads, unsigned blocks)
+{
+    if(!g) return ST_NOT_CREATED;
+    const unsigned long long size = (unsigned long long)threads * blocks;
+    if(size == 0 || size % 256 != 0 || size > (1u << 22))
+        return ST_OUT_OF_RANGE;
+    if(is_counter(g->type))
+        g->ctr_stride = (unsigned)size;
+    else if(is_interleaved(g->type))
+        g->S = (unsigned)size;
+    else
+        return ST_TYPE_ERROR;
+    g->initialized = 0;
     return ST_SUCCESS;
 }
 
@@ -826,9 +851,10 @@ static void gen_counter(orc_gen* g, const dist* d, uint8_t* out, size_t n, uint6
     const size_t   vec_n = (n - head) / full;
     const int      philox = g->type == RNG_PHILOX4_32_10;
     uint8_t        tmp[64];
-    for(uint32_t t = 0; t < CTR_STRIDE; t++)
+    const uint32_t stride = g->ctr_stride;
+    for(uint32_t t = 0; t < stride; t++)
     {
-        const int active = (t < vec_n) || (t == 0 && head > 0) || (t == vec_n % CTR_STRIDE && tail > 0);
+        const int active = (t < vec_n) || (t == 0 && head > 0) || (t == vec_n % stride && tail > 0);
         if(!active)
             continue;
         ctr_engine     e = g->ce;
@@ -854,10 +880,10 @@ static void gen_counter(orc_gen* g, const dist* d, uint8_t* out, size_t n, uint6
         while(index < vec_n)
         {
             uint64_t v[4];
-            ce_next_leap(&e, CTR_STRIDE, v);
+            ce_next_leap(&e, stride, v);
             for(unsigned s = 0; s < opt; s++)
                 d->apply(d, v + s * in_w, out + (mis + index * full + (size_t)s * out_w) * esz);
-            index += CTR_STRIDE;
+            index += stride;
         }
         if(index == vec_n && tail > 0)
         {
@@ -897,7 +923,7 @@ static uint32_t inter_next(int type, void* engines, unsigned eid)
  * reference's per-engine digit expansion. */
 static void init_interleaved(orc_gen* g)
 {
-    const unsigned   S = S_ENGINES;
+    const unsigned   S = g->S;
     const uint64_t   base_off = g->offset / S;
     g->start = (unsigned)(g->offset % S);
     free(g->engines);
@@ -953,7 +979,7 @@ static void init_interleaved(orc_gen* g)
 /* mrg.hpp:65-157 generate_mrg (same body: xorwow.hpp:63-161, lfsr113.hpp:66-160). */
 static void gen_interleaved(orc_gen* g, const dist* d, uint8_t* out, size_t n, uint64_t addr)
 {
-    const unsigned S = S_ENGINES;
+    const unsigned S = g->S;
     const unsigned in_w = d->in_w, out_w = d->out_w, esz = d->esz;
     const size_t   mis = (out_w - (addr / esz) % out_w) % out_w;
     const unsigned head = (unsigned)(n < mis ? n : mis);

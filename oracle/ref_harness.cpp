@@ -127,6 +127,9 @@ struct gen_base
     uint4              seed4  = {2, 8, 16, 128};
     unsigned long long offset = 0;
     bool               initialized = false;
+    // threads x blocks of the generate kernels' grid (0: the DEFAULT config); the kernels
+    // run with 256-thread blocks, so only the product matters
+    unsigned           grid = 0;
     virtual ~gen_base() = default;
     virtual int generate(int kind, void* out, size_t n, uint64_t addr, double a, double b) = 0;
 };
@@ -160,10 +163,11 @@ struct counter_gen : gen_base
         aligned_out tmp(n * sizeof(T), addr);
         T*          data = reinterpret_cast<T*>(tmp.p);
         // config_defaults.hpp:54-108: <<<1024, 256>>> for Philox and every ThreeFry
+        const unsigned blocks = grid ? grid / 256 : 1024;
         if constexpr(IsPhilox)
-            launch(1024, 256, generate_philox<T, D>, engine, data, n, dist);
+            launch(blocks, 256, generate_philox<T, D>, engine, data, n, dist);
         else
-            launch(1024, 256, generate_threefry<Engine, T, D>, engine, data, n, dist);
+            launch(blocks, 256, generate_threefry<Engine, T, D>, engine, data, n, dist);
         std::memcpy(out, data, n * sizeof(T));
         engine.discard((n + D::output_width - 1) / D::output_width * D::input_width);
         return 0;
@@ -223,13 +227,15 @@ struct counter_gen : gen_base
 template<class Engine, rocrand_rng_type Type>
 struct interleaved_gen : gen_base
 {
-    static constexpr unsigned S = 131072; // 512 x 256, config_defaults.hpp:33-73
+    static constexpr unsigned S_DEFAULT = 131072; // 512 x 256, config_defaults.hpp:33-73
     std::vector<Engine>       engines;
     unsigned                  start = 0;
+    unsigned                  S     = S_DEFAULT;
     using CP                  = default_config_provider<Type>;
 
     void init()
     {
+        S     = grid ? grid : S_DEFAULT;
         start = static_cast<unsigned>(offset % S);
         engines.assign(S, Engine());
         if constexpr(Type == ROCRAND_RNG_PSEUDO_XORWOW)
@@ -250,11 +256,11 @@ struct interleaved_gen : gen_base
         aligned_out tmp(n * sizeof(T), addr);
         T*          data = reinterpret_cast<T*>(tmp.p);
         if constexpr(Type == ROCRAND_RNG_PSEUDO_XORWOW)
-            launch(512, 256, generate_xorwow<CP, false, T, D>, engines.data(), start, data, n, dist);
+            launch(S / 256, 256, generate_xorwow<CP, false, T, D>, engines.data(), start, data, n, dist);
         else if constexpr(Type == ROCRAND_RNG_PSEUDO_LFSR113)
-            launch(512, 256, generate_lfsr113<CP, false, T, D>, engines.data(), start, data, n, dist);
+            launch(S / 256, 256, generate_lfsr113<CP, false, T, D>, engines.data(), start, data, n, dist);
         else
-            launch(512, 256, generate_mrg<CP, false, Engine, T, D>, engines.data(), start, data, n, dist);
+            launch(S / 256, 256, generate_mrg<CP, false, Engine, T, D>, engines.data(), start, data, n, dist);
         std::memcpy(out, data, n * sizeof(T));
         start = static_cast<unsigned>((start + (n + D::output_width - 1) / D::output_width) % S);
         return 0;
@@ -752,6 +758,29 @@ int ref_mt19937_set_limit(void* p, unsigned int limit)
 const unsigned int* ref_scrambled_sobol32_vectors() { return rocrand_h_scrambled_sobol32_direction_vectors; }
 const unsigned int* ref_scramble32_constants() { return h_scrambled_sobol32_constants; }
 const unsigned long long* ref_scramble64_constants() { return h_scrambled_sobol64_constants; }
+
+// A tuned launch grid (threads x blocks, a multiple of 256) for the counter and
+// interleaved engines, as a DYNAMIC-ordering config would select (config_types.hpp:314-336).
+int ref_set_grid(void* p, unsigned threads, unsigned blocks)
+{
+    auto*                    g    = static_cast<gen_base*>(p);
+    const unsigned long long size = (unsigned long long)threads * blocks;
+    if(size == 0 || size % 256 != 0 || size > (1u << 22))
+        return ROCRAND_STATUS_OUT_OF_RANGE;
+    switch(g->type)
+    {
+        case ROCRAND_RNG_PSEUDO_MTGP32:
+        case ROCRAND_RNG_PSEUDO_MT19937:
+        case ROCRAND_RNG_QUASI_SOBOL32:
+        case ROCRAND_RNG_QUASI_SOBOL64:
+        case ROCRAND_RNG_QUASI_SCRAMBLED_SOBOL32:
+        case ROCRAND_RNG_QUASI_SCRAMBLED_SOBOL64: return ROCRAND_STATUS_TYPE_ERROR;
+        default: break;
+    }
+    g->grid        = (unsigned)size;
+    g->initialized = false;
+    return 0;
+}
 
 int ref_set_offset(void* p, unsigned long long offset)
 {

@@ -2,6 +2,7 @@
 //
 // Every entry point keeps the reference's argument meaning and status codes
 // (library/src/rocrand.cpp:186-630, generator templates in library/src/rng/).
+#include "rr_dynamic.hpp"
 #include "rr_generator.hpp"
 
 #include <rocrand.h>
@@ -495,6 +496,16 @@ public:
     {
         if(m_initialized)
             return ROCRAND_STATUS_SUCCESS;
+        // the reference kernel's grid stride under this ordering (config_types.hpp:314-336)
+        m_stride = CTR_REF_STRIDE;
+        if(order == ROCRAND_ORDERING_PSEUDO_DYNAMIC)
+        {
+            unsigned       threads, blocks;
+            rocrand_status st = dynamic_grid(type, threads, blocks);
+            if(st != ROCRAND_STATUS_SUCCESS)
+                return st;
+            m_stride = threads * blocks;
+        }
         // engine(seed, subsequence 0, offset): the stream starts at word `offset`.
         m_pos         = offset;
         m_initialized = true;
@@ -567,9 +578,10 @@ public:
         c.pos       = to_pos(m_pos);
         c.vec_base  = to_pos(m_pos + H);
         c.vec_base0 = to_pos(m_pos + C0);
-        c.quirk     = (H != C0) ? 1 : 0;
-        c.data      = data;
-        c.n         = n;
+        c.quirk      = (H != C0) ? 1 : 0;
+        c.ref_stride = m_stride;
+        c.data       = data;
+        c.n          = n;
     }
 
 protected:
@@ -603,6 +615,7 @@ private:
     unsigned long long m_seed        = 0;
     bool               m_initialized = false;
     unsigned __int128  m_pos         = 0;
+    uint32_t           m_stride      = CTR_REF_STRIDE;
 };
 
 // ============================================================================
@@ -666,6 +679,25 @@ public:
     {
         if(m_initialized)
             return ROCRAND_STATUS_SUCCESS;
+        // engine count: the reference's grid under this ordering (mrg.hpp:284-337)
+        uint32_t engines = INTERLEAVED_ENGINES;
+        if(order == ROCRAND_ORDERING_PSEUDO_DYNAMIC)
+        {
+            unsigned       threads, blocks;
+            rocrand_status st = dynamic_grid(type, threads, blocks);
+            if(st != ROCRAND_STATUS_SUCCESS)
+                return st;
+            engines = threads * blocks;
+        }
+        if(m_states && engines != m_S)
+        {
+            // earlier calls' kernels on the stream may still read the old states
+            if(hipStreamSynchronize(stream) != hipSuccess)
+                return ROCRAND_STATUS_INTERNAL_ERROR;
+            (void)hipFree(m_states);
+            m_states = nullptr;
+        }
+        m_S = engines;
         if(!m_states)
         {
             // two state buffers: a call reads one and leaves the advanced states in the other
@@ -674,12 +706,12 @@ public:
                 m_states = nullptr;
                 return ROCRAND_STATUS_ALLOCATION_FAILED;
             }
+            m_cur = 0;
         }
-        m_start = (uint32_t)(offset % INTERLEAVED_ENGINES);
+        m_start = (uint32_t)(offset % m_S);
         uint32_t seeded[6];
         seed_state(seeded);
-        if(launch_interleaved_init(m_kind, cur_states(), seeded, m_start, offset / INTERLEAVED_ENGINES, stream)
-           != hipSuccess)
+        if(launch_interleaved_init(m_kind, cur_states(), seeded, m_start, offset / m_S, m_S, stream) != hipSuccess)
             return ROCRAND_STATUS_LAUNCH_FAILURE;
         m_initialized = true;
         return ROCRAND_STATUS_SUCCESS;
@@ -744,13 +776,14 @@ protected:
         c.states     = cur_states();
         c.states_out = m_states + (m_cur ^ 1u) * state_words();
         c.start      = m_start;
+        c.S          = m_S;
         c.data   = data;
         c.n      = n;
         if(launch_interleaved(m_kind, k, c, stream) != hipSuccess)
             return ROCRAND_STATUS_LAUNCH_FAILURE;
         m_cur ^= 1u;
         // mrg.hpp:385-389
-        m_start = (uint32_t)((m_start + (n + s.out_w - 1) / s.out_w) % INTERLEAVED_ENGINES);
+        m_start = (uint32_t)((m_start + (n + s.out_w - 1) / s.out_w) % m_S);
         return ROCRAND_STATUS_SUCCESS;
     }
 
@@ -762,8 +795,9 @@ private:
     uint32_t*          m_states      = nullptr; // 2 * state_words() words
     uint32_t           m_cur         = 0;       // which half holds the current states
     uint32_t           m_start       = 0;
+    uint32_t           m_S           = INTERLEAVED_ENGINES; // engines
 
-    size_t    state_words() const { return (size_t)inter_state_words(m_kind) * INTERLEAVED_ENGINES; }
+    size_t    state_words() const { return (size_t)inter_state_words(m_kind) * m_S; }
     uint32_t* cur_states() const { return m_states + m_cur * state_words(); }
 };
 

@@ -139,7 +139,7 @@ __device__ void head_tail(const counter_call& c)
         if(c.tail > 0)
         {
             // Tail: the thread whose next vec index is vec_n continues from that vec's position.
-            const ctr_pos& b = (c.quirk && (c.vec_n % CTR_REF_STRIDE) == 0) ? c.vec_base0 : c.vec_base;
+            const ctr_pos& b = (c.quirk && (c.vec_n % c.ref_stride) == 0) ? c.vec_base0 : c.vec_base;
             seq_reader<K>  r(b, c.vec_n, c.key);
             for(int s = 0; s < OPT; s++)
             {
@@ -177,7 +177,7 @@ __device__ __forceinline__ void generic_vec(const counter_call& c, size_t v)
     static_assert(BYTES == 8 || BYTES == 16 || BYTES == 32, "vec size");
 
     out_t* const   vec_data = static_cast<out_t*>(c.data) + c.mis;
-    const ctr_pos& b        = (c.quirk && (v % CTR_REF_STRIDE) == 0) ? c.vec_base0 : c.vec_base;
+    const ctr_pos& b        = (c.quirk && (v % c.ref_stride) == 0) ? c.vec_base0 : c.vec_base;
     W              ctr[DIM];
 #pragma unroll
     for(int i = 0; i < DIM; i++)

@@ -23,14 +23,14 @@ using namespace inter_dev;
 
 template<inter_kind K>
 __global__ __launch_bounds__(256) void init_kernel(uint32_t* states, const device_tables t, ieng<K> seeded,
-                                                   uint32_t start, uint64_t base_off)
+                                                   uint32_t start, uint64_t base_off, uint32_t ns)
 {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if(e >= S)
+    if(e >= ns)
         return;
     ieng<K> g = seeded;
     g.init(t, e, base_off + (e < start ? 1u : 0u));
-    g.store(states, e);
+    g.store(states, e, ns);
 }
 
 // Poisson: each block copies the packed alias table into dynamic LDS (size * 8 bytes, no
@@ -66,12 +66,13 @@ __global__ __launch_bounds__(256) void generate_kernel(interleaved_call c)
     static_assert(BYTES == 4 || BYTES == 8 || BYTES == 16, "vec size");
 
     stage_poisson_table<D>(c);
+    const uint32_t S  = c.S;
     const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if(id >= S)
         return;
     const uint32_t eid = (id + c.start) % S;
     ieng<K>        g;
-    g.load(c.states, eid);
+    g.load(c.states, eid, S);
 
     out_t* const data     = static_cast<out_t*>(c.data);
     out_t* const vec_data = data + c.mis;
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(256) void generate_kernel(interleaved_call c)
     const uint32_t     id_last = blockIdx.x * blockDim.x + blockDim.x - 1;
     const uint32_t     ucnt    = id_last < c.vec_n ? (uint32_t)((c.vec_n - id_last + S - 1) / S) : 0u;
     const uint32_t     off     = id * (uint32_t)BYTES;
-    constexpr uint32_t lim     = S * (uint32_t)BYTES;
+    const uint32_t     lim     = S * (uint32_t)BYTES;
     char*              base    = reinterpret_cast<char*>(vec_data);
 #pragma unroll UNROLL
     for(uint32_t k = 0; k < ucnt; k++, index += S, base += (size_t)S * BYTES)
@@ -138,7 +139,7 @@ __global__ __launch_bounds__(256) void generate_kernel(interleaved_call c)
                     data[c.n - c.tail + j] = o[j];
         }
     }
-    g.store(c.states_out, eid);
+    g.store(c.states_out, eid, S);
 }
 
 // Long calls: each engine's vec-steps are split into J chunks of C, one lane per
@@ -163,7 +164,8 @@ __global__ __launch_bounds__(256) void generate_chunked_kernel(interleaved_call 
 
     // S is a multiple of the block size: the chunk index is uniform per block, so the
     // jump's digit loop and its table rows stay in scalar registers
-    constexpr uint32_t BPC = S / 256; // blocks per chunk
+    const uint32_t     S   = c.S;
+    const uint32_t     BPC = S / 256; // blocks per chunk
     const uint32_t     j   = blockIdx.x / BPC;
     const uint32_t     id  = (blockIdx.x % BPC) * 256 + threadIdx.x;
     if(j >= J)
@@ -182,7 +184,7 @@ __global__ __launch_bounds__(256) void generate_chunked_kernel(interleaved_call 
         // GF(2) jumps: k0 is the same for the whole block, which stages each matrix in
         // LDS once; every thread takes part (barriers) before idle ones leave
         __shared__ uint32_t mat[ieng<K>::LDS_WORDS];
-        eng.load(c.states, eid);
+        eng.load(c.states, eid, S);
         eng.advance_block(t, k0 * IN, mat);
         if(idle)
             return;
@@ -191,7 +193,7 @@ __global__ __launch_bounds__(256) void generate_chunked_kernel(interleaved_call 
     {
         if(idle)
             return;
-        eng.load(c.states, eid);
+        eng.load(c.states, eid, S);
         eng.advance(t, k0 * IN);
     }
 
@@ -257,12 +259,13 @@ __global__ __launch_bounds__(256) void generate_chunked_kernel(interleaved_call 
                     data[c.n - c.tail + q] = o[q];
         }
     }
-    eng.store(c.states_out, eid);
+    eng.store(c.states_out, eid, S);
 }
 
 template<inter_kind K, class D>
 hipError_t launch_gen(const interleaved_call& c, hipStream_t s)
 {
+    const uint32_t S = c.S;
     if constexpr(K == inter_kind::xorwow || K == inter_kind::lfsr113)
     {
         // XORWOW / LFSR113 write cheap outputs at the store rate with S lanes, and the
@@ -338,7 +341,7 @@ hipError_t dispatch(out_kind o, const interleaved_call& c, hipStream_t s)
 
 template<inter_kind K>
 hipError_t launch_init_k(uint32_t* states, const uint32_t* seed_state, uint32_t start, uint64_t base_off,
-                         hipStream_t s)
+                         uint32_t ns, hipStream_t s)
 {
     device_tables t;
     hipError_t    err = get_device_tables(t);
@@ -346,27 +349,36 @@ hipError_t launch_init_k(uint32_t* states, const uint32_t* seed_state, uint32_t 
         return err;
     ieng<K> seeded;
     seeded.set(seed_state);
-    hipLaunchKernelGGL((init_kernel<K>), dim3(S / 256), dim3(256), 0, s, states, t, seeded, start, base_off);
+    hipLaunchKernelGGL((init_kernel<K>), dim3((ns + 255) / 256), dim3(256), 0, s, states, t, seeded, start, base_off,
+                       ns);
     return hipGetLastError();
 }
 
 } // namespace
 
 hipError_t launch_interleaved_init(inter_kind k, uint32_t* states, const uint32_t* seed_state, uint32_t start,
-                                   uint64_t base_offset, hipStream_t s)
+                                   uint64_t base_offset, uint32_t engines, hipStream_t s)
 {
+    if(engines == 0 || engines % 256 != 0)
+        return hipErrorInvalidValue;
     switch(k)
     {
-        case inter_kind::mrg32k3a: return launch_init_k<inter_kind::mrg32k3a>(states, seed_state, start, base_offset, s);
-        case inter_kind::mrg31k3p: return launch_init_k<inter_kind::mrg31k3p>(states, seed_state, start, base_offset, s);
-        case inter_kind::xorwow: return launch_init_k<inter_kind::xorwow>(states, seed_state, start, base_offset, s);
-        case inter_kind::lfsr113: return launch_init_k<inter_kind::lfsr113>(states, seed_state, start, base_offset, s);
+        case inter_kind::mrg32k3a:
+            return launch_init_k<inter_kind::mrg32k3a>(states, seed_state, start, base_offset, engines, s);
+        case inter_kind::mrg31k3p:
+            return launch_init_k<inter_kind::mrg31k3p>(states, seed_state, start, base_offset, engines, s);
+        case inter_kind::xorwow:
+            return launch_init_k<inter_kind::xorwow>(states, seed_state, start, base_offset, engines, s);
+        case inter_kind::lfsr113:
+            return launch_init_k<inter_kind::lfsr113>(states, seed_state, start, base_offset, engines, s);
     }
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_interleaved(inter_kind k, out_kind o, const interleaved_call& c, hipStream_t s)
 {
+    if(c.S == 0 || c.S % 256 != 0)
+        return hipErrorInvalidValue;
     switch(k)
     {
         case inter_kind::mrg32k3a: return dispatch<inter_kind::mrg32k3a>(o, c, s);

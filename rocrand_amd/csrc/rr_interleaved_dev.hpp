@@ -11,6 +11,8 @@ namespace rr
 namespace inter_dev
 {
 
+// Engines of the DEFAULT grid; the per-call kernels take the call's engine count (the
+// DYNAMIC grid may differ), the batched kernels use this one.
 constexpr uint32_t S = INTERLEAVED_ENGINES;
 
 template<inter_kind K>
@@ -35,14 +37,16 @@ __device__ __forceinline__ void mrg_entry_apply(const uint32_t* entry, mrg_state
     mat3_vec_fold<K, C2>(w + 9, s.g2);
 }
 
-// Jump to subsequence e (e < 2^17) with the radix tables: X^(e mod 512), then
-// X^(512 (e / 512)); the powers commute.  Same state as the binary expansion over the
-// A^(2^(sub + i)) table of the reference's skipahead_subsequence.
+// Jump to subsequence e mod 2^17 with the radix tables: X^(e mod 512), then
+// X^(512 ((e / 512) mod 256)); the powers commute.  Same state as the binary expansion
+// over the A^(2^(sub + i)) table of the reference's skipahead_subsequence.  (Engines at
+// or beyond 2^17, which only a larger DYNAMIC grid has, add their high part by that
+// binary expansion: see ieng::sub.)
 template<uint32_t K, uint32_t C1, uint32_t C2>
 __device__ __forceinline__ void mrg_sub_jump(const uint32_t* tab, uint32_t e, mrg_state& s)
 {
     const uint32_t lo = e & (uint32_t)(MRG_RADIX_LO - 1);
-    const uint32_t hi = e >> MRG_RADIX_LO_BITS;
+    const uint32_t hi = (e >> MRG_RADIX_LO_BITS) & (uint32_t)(MRG_RADIX_HI - 1);
     if(lo != 0u)
         mrg_entry_apply<K, C1, C2>(tab + lo * MRG_RADIX_WORDS, s);
     if(hi != 0u)
@@ -54,22 +58,22 @@ struct ieng<inter_kind::mrg32k3a>
 {
     static constexpr word_family F = word_family::mrg32;
     mrg_state                    s;
-    __device__ void load(const uint32_t* st, uint32_t e)
+    __device__ void load(const uint32_t* st, uint32_t e, uint32_t ns = S)
     {
 #pragma unroll
         for(int j = 0; j < 3; j++)
         {
-            s.g1[j] = st[j * S + e];
-            s.g2[j] = st[(3 + j) * S + e];
+            s.g1[j] = st[j * ns + e];
+            s.g2[j] = st[(3 + j) * ns + e];
         }
     }
-    __device__ void store(uint32_t* st, uint32_t e) const
+    __device__ void store(uint32_t* st, uint32_t e, uint32_t ns = S) const
     {
 #pragma unroll
         for(int j = 0; j < 3; j++)
         {
-            st[j * S + e]       = s.g1[j];
-            st[(3 + j) * S + e] = s.g2[j];
+            st[j * ns + e]       = s.g1[j];
+            st[(3 + j) * ns + e] = s.g2[j];
         }
     }
     __host__ __device__ void set(const uint32_t* w)
@@ -91,7 +95,12 @@ struct ieng<inter_kind::mrg32k3a>
                 mat3_vec_fold<32, 22853>(tab + (i * 2 + 1) * 9, s.g2);
             }
     }
-    __device__ void sub(const device_tables& t, uint32_t e) { mrg_sub_jump<32, 209, 22853>(t.mrg32_subr, e, s); }
+    __device__ void sub(const device_tables& t, uint32_t e)
+    {
+        mrg_sub_jump<32, 209, 22853>(t.mrg32_subr, e, s);
+        if(e >> 17) // A^(2^(76+17+i)) rows of the subsequence table
+            jump(t.mrg32_sub + 17 * 2 * 9, e >> 17);
+    }
     // k steps ahead (rocrand_mrg32k3a.h:246-263)
     __device__ void advance(const device_tables& t, uint64_t k) { jump(t.mrg32_step, k); }
     __device__ void init(const device_tables& t, uint32_t e, uint64_t off)
@@ -115,7 +124,12 @@ struct ieng<inter_kind::mrg31k3p> : ieng<inter_kind::mrg32k3a>
                 mat3_vec_fold<31, 21069>(tab + (i * 2 + 1) * 9, s.g2);
             }
     }
-    __device__ void sub(const device_tables& t, uint32_t e) { mrg_sub_jump<31, 1, 21069>(t.mrg31_subr, e, s); }
+    __device__ void sub(const device_tables& t, uint32_t e)
+    {
+        mrg_sub_jump<31, 1, 21069>(t.mrg31_subr, e, s);
+        if(e >> 17) // A^(2^(72+17+i)) rows of the subsequence table
+            jump(t.mrg31_sub + 17 * 2 * 9, e >> 17);
+    }
     __device__ void advance(const device_tables& t, uint64_t k) { jump(t.mrg31_step, k); }
     __device__ void init(const device_tables& t, uint32_t e, uint64_t off)
     {
@@ -164,19 +178,19 @@ struct ieng<inter_kind::xorwow>
 {
     static constexpr word_family F = word_family::w32;
     xorwow_state                 s;
-    __device__ void load(const uint32_t* st, uint32_t e)
+    __device__ void load(const uint32_t* st, uint32_t e, uint32_t ns = S)
     {
         s.d = st[e];
 #pragma unroll
         for(int j = 0; j < 5; j++)
-            s.x[j] = st[(1 + j) * S + e];
+            s.x[j] = st[(1 + j) * ns + e];
     }
-    __device__ void store(uint32_t* st, uint32_t e) const
+    __device__ void store(uint32_t* st, uint32_t e, uint32_t ns = S) const
     {
         st[e] = s.d;
 #pragma unroll
         for(int j = 0; j < 5; j++)
-            st[(1 + j) * S + e] = s.x[j];
+            st[(1 + j) * ns + e] = s.x[j];
     }
     __host__ __device__ void set(const uint32_t* w)
     {
@@ -211,17 +225,17 @@ struct ieng<inter_kind::lfsr113>
 {
     static constexpr word_family F = word_family::w32;
     lfsr113_state                s;
-    __device__ void load(const uint32_t* st, uint32_t e)
+    __device__ void load(const uint32_t* st, uint32_t e, uint32_t ns = S)
     {
 #pragma unroll
         for(int j = 0; j < 4; j++)
-            s.z[j] = st[j * S + e];
+            s.z[j] = st[j * ns + e];
     }
-    __device__ void store(uint32_t* st, uint32_t e) const
+    __device__ void store(uint32_t* st, uint32_t e, uint32_t ns = S) const
     {
 #pragma unroll
         for(int j = 0; j < 4; j++)
-            st[j * S + e] = s.z[j];
+            st[j * ns + e] = s.z[j];
     }
     __host__ __device__ void set(const uint32_t* w)
     {

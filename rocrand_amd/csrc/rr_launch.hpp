@@ -25,8 +25,10 @@ struct counter_call
     ctr_key  key;
     ctr_pos  pos;        // word position of the call's first word (engine state)
     ctr_pos  vec_base;   // position of vec 0's first word: pos + H
-    ctr_pos  vec_base0;  // position used by vec indices that are multiples of CTR_REF_STRIDE: pos + C0
+    ctr_pos  vec_base0;  // position used by vec indices that are multiples of ref_stride: pos + C0
     int      quirk;      // vec_base0 != vec_base
+    uint32_t ref_stride; // grid stride of the reference's generate kernel for the ordering
+                         // (CTR_REF_STRIDE, or the DYNAMIC grid)
     void*    data;       // output base pointer
     size_t   n;          // number of outputs
     size_t   mis;        // misalignment (outputs before the first full vec)
@@ -44,6 +46,7 @@ struct interleaved_call
     uint32_t*   states_out; // where the call leaves the advanced states (same layout; a second
                             // buffer, so no block of a launch reads a state another block writes)
     uint32_t    start;    // start engine
+    uint32_t    S;        // engines (INTERLEAVED_ENGINES, or the DYNAMIC grid), a multiple of 256
     void*       data;
     size_t      n, mis;
     uint32_t    head, tail;
@@ -70,6 +73,7 @@ hipError_t launch_interleaved_init(inter_kind k,
                                    const uint32_t* seed_state, // one engine's seeded state
                                    uint32_t   start,
                                    uint64_t   base_offset,
+                                   uint32_t   engines, // S
                                    hipStream_t s);
 hipError_t launch_interleaved(inter_kind k, out_kind o, const interleaved_call& c, hipStream_t s);
 

@@ -116,6 +116,9 @@ def load():
     L.rocrand_batch_plan_destroy.argtypes = [vp]
     L.rocrand_generate_batched.argtypes = [ctypes.POINTER(BatchStream), sz, vp]
     L.rocrand_generate_batched_noalloc.argtypes = [ctypes.POINTER(BatchStream), sz, vp]
+    L.rocrand_amd_set_dynamic_grid.argtypes = [ctypes.c_int, ctypes.c_uint, ctypes.c_uint]
+    L.rocrand_amd_get_dynamic_grid.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint),
+                                               ctypes.POINTER(ctypes.c_uint)]
     _lib = L
     return L
 
@@ -189,6 +192,19 @@ class RocRandError(Exception):
 def check_rocrand(status):
     if status != 0:
         raise RocRandError(status)
+
+
+def set_dynamic_grid(rngtype, threads, blocks):
+    """rocrand_amd_set_dynamic_grid (include/rocrand_amd_tuning.h): the DYNAMIC-ordering grid of
+    an engine in this process; (0, 0) restores the library's table."""
+    check_rocrand(load().rocrand_amd_set_dynamic_grid(rngtype, threads, blocks))
+
+
+def get_dynamic_grid(rngtype):
+    """(threads, blocks) a DYNAMIC-ordering generator of this engine uses on the current device."""
+    t, b = ctypes.c_uint(), ctypes.c_uint()
+    check_rocrand(load().rocrand_amd_get_dynamic_grid(rngtype, ctypes.byref(t), ctypes.byref(b)))
+    return t.value, b.value
 
 
 def get_version():

@@ -339,3 +339,43 @@ def test_engine_streams_equal_reference(rng):
         a = O.engine_stream(rng, seed, sub, off, 256)
         b = O.engine_stream(rng, seed, sub, off, 256, use_ref=True)
         assert np.array_equal(a, b)
+
+
+# Tuned launch grids (DYNAMIC ordering): the engine count of the interleaved engines and
+# the counter engines' grid stride follow threads x blocks (config_types.hpp:314-336,
+# mrg.hpp:284-337, philox4x32_10.hpp:106-214).  Sizes where the grid matters: enough
+# outputs to wrap the engine count or the stride, misaligned and odd lengths.
+GRIDS = [(256, 256), (512, 512)]
+
+
+@ref_only
+@pytest.mark.parametrize("rng", COUNTER + INTERLEAVED)
+@pytest.mark.parametrize("grid", GRIDS)
+def test_oracle_grid_equals_reference_kernel_bodies(rng, grid):
+    size = grid[0] * grid[1]
+    for kind, seed, off, n, shift in [("u32", 12345, 1048583, size + 77, 1),
+                                      ("normal_f32", 7, 3 * size + 5, size + 3, 2),
+                                      ("uniform_f64", 99, 17, 2 * size + 1, 1),
+                                      ("u32", 3, 0, 4 * size + 13, 3)]:  # past the stride's multiples
+        addr = 0x1000 + shift * np.dtype(O.KINDS[kind][1]).itemsize
+        g1 = O.OracleGen(rng, seed=seed, offset=off, grid=grid)
+        g2 = O.RefGen(rng, seed=seed, offset=off, grid=grid)
+        for _ in range(2):
+            s1, x1 = g1.generate(kind, n, addr, 0.25, 1.5)
+            s2, x2 = g2.generate(kind, n, addr, 0.25, 1.5)
+            assert s1 == s2 == 0
+            assert x1.tobytes() == x2.tobytes(), (rng, grid, kind, seed)
+
+
+@pytest.mark.parametrize("rng", INTERLEAVED)
+def test_oracle_grid_changes_the_interleaving(rng):
+    """A different engine count is a different stream (the grid is not value-neutral for
+    the interleaved engines), and the default grid's output is unchanged by set_grid with
+    the default (512 x 256)."""
+    n = 300000
+    _, a = O.OracleGen(rng, seed=5).generate("u32", n)
+    _, b = O.OracleGen(rng, seed=5, grid=(512, 256)).generate("u32", n)
+    _, c = O.OracleGen(rng, seed=5, grid=(256, 1024)).generate("u32", n)
+    assert np.array_equal(a, b)
+    assert np.array_equal(a[:131072], c[:131072])  # engines below 131072 agree at step 0
+    assert not np.array_equal(a[131072:262144], c[131072:262144])
