@@ -212,11 +212,12 @@ def batched_bench(R, torch, count=4096, seed=7, launches=10):
 def serving_bench(torch, rates=(1000, 10000, 50000), seconds=1.0, budget_gb=32):
     """Open-loop Poisson arrivals at each rate for `seconds` (rocrand_amd.workloads.serving_mix:
     sizes log-uniform in [256, 2^26], 50% Philox / 30% MRG32k3a / 20% Sobol64 x16 dims,
-    f32/f64/u32) through the continuous-batching server with a 32 GB HBM budget."""
+    f32/f64/u32) through the request server (include/rocrand_serving.h) with a 32 GB HBM
+    budget and two streams."""
     from rocrand_amd.serving import Server
     from rocrand_amd.workloads import serving_mix
 
-    srv = Server(torch, int(budget_gb * 1e9))
+    srv = Server(int(budget_gb * 1e9), streams=2)
     # warm-up: kernels, tables
     srv.serve(serving_mix(50, 1e6, seed=1))
     out = []
@@ -225,11 +226,10 @@ def serving_bench(torch, rates=(1000, 10000, 50000), seconds=1.0, budget_gb=32):
         res = srv.serve(reqs)
         out.append({"rate_rps": rate, "requests": len(reqs), "p50_ms": round(res.percentile(50) * 1e3, 3),
                     "p99_ms": round(res.percentile(99) * 1e3, 3), "GBps": round(res.bytes / res.wall_s / 1e9, 1),
-                    "iterations": res.iterations, "max_batch": res.max_batch, "max_inflight": res.max_inflight,
-                    "rejected": res.rejected,
+                    "polls": res.polls, "rejected": res.rejected,
                     "wall_s": round(res.wall_s, 3)})
         print(f"# serving: {out[-1]}", file=sys.stderr, flush=True)
-    del srv
+    srv.close()
     return {"budget_GB": budget_gb, "workload_seed": 11, "runs": out}
 
 

@@ -1,35 +1,69 @@
-"""Continuous-batching request server over the batched C ABI.
+"""Continuous-batching request server: Python client of include/rocrand_serving.h.
 
-Requests (rocrand_amd.workloads.Request) arrive open-loop at their timestamps.  The
-server loop never waits for the device while work is pending:
+The server itself is C++ (rocrand_amd/csrc/rr_serving.cpp): its HBM arena (first fit with
+coalescing over one allocation), FIFO admission, batch issue on alternating streams,
+completion events and session generators.  This module binds it and adds the open-loop
+driver used by the benchmark: requests (rocrand_amd.workloads.Request) become visible at
+their arrival timestamps, every turn submits the new arrivals and polls once, and a
+request's latency is the time its completion was reported minus its arrival time.  The
+host never waits for the device while requests are pending.
 
-* every turn it retires the batches whose completion event has fired (their outputs
-  are done; their memory returns to the arena) and timestamps their requests;
-* it then admits the queued requests, first come first served, whose outputs fit the
-  free arena memory (one pre-allocated HBM budget, first-fit with coalescing; a request
-  larger than the whole budget is rejected), and issues them as one batch: one
-  asynchronous batched call (rocrand_generate_batched, include/rocrand_batched.h) for the
-  Philox4x32-10 / MRG32k3a requests, one quasi-random generator call per Sobol64
-  request, then a completion event;
-* consecutive batches alternate between two device streams, so a batch admitted while
-  earlier ones run overlaps with them on the device.
-
-Latency = time the completion was observed - arrival time.  A request's output does
-not depend on its batch: it is the output of the plain call sequence of
-include/rocrand_batched.h (Sobol64: a 16-dimensional generator at offset seed mod 2^20).
+A request's output is that of a plain call sequence (include/rocrand_serving.h): session 0
+is a fresh generator's single call (Sobol64: 'dims' dimensions at offset seed mod 2^20);
+a session s != 0 continues the generator of session s.
 """
 from __future__ import annotations
 
-import bisect
-import collections
+import ctypes
 import time
 from dataclasses import dataclass, field
 
+import numpy as np
+
 from . import rocrand as R
 
-KIND = {"f32": R.BATCH_UNIFORM_FLOAT, "f64": R.BATCH_UNIFORM_DOUBLE, "u32": R.BATCH_UINT}
+DTYPE = {"u32": 0, "f32": 1, "f64": 2}
 ITEMSIZE = {"f32": 4, "f64": 8, "u32": 4}
+NUMPY = {"f32": np.float32, "f64": np.float64, "u32": np.uint32}
 ALIGN = 256
+
+
+class ServeRequest(ctypes.Structure):
+    _fields_ = [("rng_type", ctypes.c_int), ("dtype", ctypes.c_int), ("seed", ctypes.c_ulonglong),
+                ("session", ctypes.c_ulonglong), ("n", ctypes.c_size_t), ("dims", ctypes.c_uint)]
+
+
+class ServeCompletion(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_ulonglong), ("output", ctypes.c_void_p), ("bytes", ctypes.c_size_t)]
+
+
+def _lib():
+    L = R.load()
+    if not getattr(L, "_serving_bound", False):
+        vp, sz, u64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_ulonglong
+        L.rocrand_server_create.argtypes = [ctypes.POINTER(vp), sz, ctypes.c_uint, ctypes.c_int]
+        L.rocrand_server_destroy.argtypes = [vp]
+        L.rocrand_server_submit.argtypes = [vp, ctypes.POINTER(ServeRequest), sz, ctypes.POINTER(u64)]
+        L.rocrand_server_poll.argtypes = [vp, ctypes.POINTER(ServeCompletion), sz, ctypes.POINTER(sz),
+                                          ctypes.POINTER(sz)]
+        L.rocrand_server_release.argtypes = [vp, u64]
+        L.rocrand_server_end_session.argtypes = [vp, u64]
+        L._serving_bound = True
+    return L
+
+
+def request_bytes(req) -> int:
+    return req.n * ITEMSIZE[req.dtype]
+
+
+def slot(nbytes: int) -> int:
+    """Arena bytes the server reserves for an output of `nbytes` (whole 256-byte units, at least one)."""
+    return max(ALIGN, (nbytes + ALIGN - 1) // ALIGN * ALIGN)
+
+
+def _check(st, what):
+    if st != 0:
+        raise R.RocRandError(st) if hasattr(R, "RocRandError") else RuntimeError(f"{what}: status {st}")
 
 
 @dataclass
@@ -37,10 +71,8 @@ class Result:
     latencies_s: list = field(default_factory=list)
     bytes: int = 0
     wall_s: float = 0.0
-    iterations: int = 0
+    polls: int = 0
     rejected: int = 0
-    max_batch: int = 0
-    max_inflight: int = 0
 
     def percentile(self, q):
         if not self.latencies_s:
@@ -50,159 +82,106 @@ class Result:
         return xs[k]
 
 
-def request_bytes(req) -> int:
-    return req.n * ITEMSIZE[req.dtype]
-
-
-def padded(nbytes: int) -> int:
-    return (nbytes + ALIGN - 1) // ALIGN * ALIGN
-
-
-def sobol_offset(req) -> int:
-    return req.seed % (1 << 20)
-
-
-def batch_streams(reqs, ptrs):
-    """BatchStream entries for the batchable requests (Philox / MRG32k3a)."""
-    out = []
-    for r, p in zip(reqs, ptrs):
-        if r.rng_type in (R.PHILOX4_32_10, R.MRG32K3A):
-            out.append(R.BatchStream(r.rng_type, KIND[r.dtype], r.seed, 0, r.n, 0.0, 0.0, p))
-    return out
-
-
-def write_sobol(req, view, stream):
-    g = R.QRNG(R.SOBOL64, ndim=req.dims, offset=sobol_offset(req))
-    g.stream = stream
-    if req.dtype == "u32":
-        g.generate(view)
-    else:
-        g.uniform(view)
-
-
-class Arena:
-    """First-fit allocator over [0, size) with coalescing of freed ranges."""
-
-    def __init__(self, size: int):
-        self.size = size
-        self.free = [(0, size)]  # sorted (start, length), no two adjacent
-
-    def alloc(self, n: int):
-        for i, (s, ln) in enumerate(self.free):
-            if ln >= n:
-                if ln == n:
-                    self.free.pop(i)
-                else:
-                    self.free[i] = (s + n, ln - n)
-                return s
-        return None
-
-    def release(self, start: int, n: int):
-        """Returns [start, start + n) to the free list, merged with its free neighbours."""
-        free = self.free
-        i = bisect.bisect_left(free, (start, 0))
-        if i < len(free) and start + n == free[i][0]:  # merge with the next range
-            n += free[i][1]
-            free.pop(i)
-        if i > 0 and free[i - 1][0] + free[i - 1][1] == start:  # and with the previous one
-            free[i - 1] = (free[i - 1][0], free[i - 1][1] + n)
-        else:
-            free.insert(i, (start, n))
-
-
-@dataclass
-class _Inflight:
-    reqs: list
-    event: object
-    regions: list
-
-
 class Server:
-    def __init__(self, torch, budget_bytes: int, streams: int = 2):
-        self.torch = torch
-        self.budget = budget_bytes
-        self.memory = torch.empty(budget_bytes, dtype=torch.uint8, device="cuda")
-        self.base = self.memory.data_ptr()
-        self.arena = Arena(budget_bytes)
-        self.streams = [torch.cuda.Stream() for _ in range(streams)]
-        self.turn = 0
+    """One rocrand_server: `budget_bytes` of HBM, `streams` HIP streams."""
 
-    def _view(self, start, req):
-        dt = {"f32": self.torch.float32, "f64": self.torch.float64, "u32": self.torch.uint32}[req.dtype]
-        return self.memory[start:start + request_bytes(req)].view(dt)
+    def __init__(self, budget_bytes: int, streams: int = 2, keep_outputs: bool = False, capacity: int = 4096):
+        self._L = _lib()
+        self._h = ctypes.c_void_p()
+        _check(self._L.rocrand_server_create(ctypes.byref(self._h), budget_bytes, streams, int(keep_outputs)),
+               "rocrand_server_create")
+        self.budget = budget_bytes // ALIGN * ALIGN
+        self._done = (ServeCompletion * capacity)()
+        self._capacity = capacity
 
-    def issue(self, reqs, regions):
-        """Queue one batch on the next stream; returns its completion event."""
-        torch = self.torch
-        stream = self.streams[self.turn % len(self.streams)]
-        self.turn += 1
-        base = self.base
-        streams = batch_streams(reqs, [base + s for s in regions])
-        with torch.cuda.stream(stream):
-            if streams:
-                R.generate_batched(streams, stream)
-            for r, s in zip(reqs, regions):
-                if r.rng_type == R.SOBOL64:
-                    write_sobol(r, self._view(s, r), stream)
-            ev = torch.cuda.Event()
-            ev.record(stream)
-        return ev
+    def close(self):
+        if self._h:
+            _check(self._L.rocrand_server_destroy(self._h), "rocrand_server_destroy")
+            self._h = ctypes.c_void_p()
 
-    def run_batch(self, reqs):
-        """Writes `reqs` as one batch and waits for it (outputs stay valid until the next call)."""
-        self.arena = Arena(self.budget)
-        regions = [self.arena.alloc(padded(request_bytes(r))) for r in reqs]
-        if any(s is None for s in regions):
-            raise ValueError("batch exceeds the budget")
-        ev = self.issue(reqs, regions)
-        views = [self._view(s, r) for s, r in zip(regions, reqs)]
-        ev.synchronize()
-        self.arena = Arena(self.budget)
-        return views
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, reqs) -> list[int]:
+        """Queues `reqs` (objects with rng_type, dtype, seed, n and optional session, dims);
+        returns their tickets (0 = rejected: larger than the budget)."""
+        if not reqs:
+            return []
+        arr = (ServeRequest * len(reqs))()
+        for i, r in enumerate(reqs):
+            arr[i] = ServeRequest(r.rng_type, DTYPE[r.dtype], r.seed & (2**64 - 1), getattr(r, "session", 0),
+                                  r.n, getattr(r, "dims", 1))
+        tickets = (ctypes.c_ulonglong * len(reqs))()
+        _check(self._L.rocrand_server_submit(self._h, arr, len(reqs), tickets), "rocrand_server_submit")
+        return list(tickets)
+
+    def poll(self):
+        """One turn: returns ([(ticket, device pointer, bytes)], pending)."""
+        k, pending = ctypes.c_size_t(), ctypes.c_size_t()
+        _check(self._L.rocrand_server_poll(self._h, self._done, self._capacity, ctypes.byref(k),
+                                           ctypes.byref(pending)), "rocrand_server_poll")
+        return [(self._done[i].ticket, self._done[i].output, self._done[i].bytes) for i in range(k.value)], \
+            pending.value
+
+    def release(self, ticket: int):
+        _check(self._L.rocrand_server_release(self._h, ticket), "rocrand_server_release")
+
+    def end_session(self, session: int):
+        _check(self._L.rocrand_server_end_session(self._h, session), "rocrand_server_end_session")
+
+    def run(self, reqs, sink=None):
+        """Submits `reqs` and polls until all are reported; `sink(index, pointer, bytes)` is
+        called for each completion while its output is valid.  Returns the tickets."""
+        tickets = self.submit(reqs)
+        index = {t: i for i, t in enumerate(tickets) if t}
+        left = len(index)
+        while left:
+            done, _ = self.poll()
+            for t, ptr, nbytes in done:
+                if sink is not None:
+                    sink(index[t], ptr, nbytes)
+                left -= 1
+            if not done:
+                time.sleep(2e-5)
+        return tickets
 
     def serve(self, requests) -> Result:
-        """Open-loop run: request i becomes visible at its arrival time."""
+        """Open-loop run: request i is submitted once its arrival time has passed."""
         res = Result()
-        i = 0
-        queue, inflight = collections.deque(), []
-        t0 = time.perf_counter()
-        while i < len(requests) or queue or inflight:
+        i, owner, t0 = 0, {}, time.perf_counter()
+        while True:
             now = time.perf_counter() - t0
-            # retire finished batches
-            still = []
-            for b in inflight:
-                if b.event.query():
-                    done = time.perf_counter() - t0
-                    for r in b.reqs:
-                        res.latencies_s.append(done - r.arrival_s)
-                        res.bytes += request_bytes(r)
-                    for s, r in zip(b.regions, b.reqs):
-                        self.arena.release(s, padded(request_bytes(r)))
-                else:
-                    still.append(b)
-            inflight = still
-            # arrivals
-            while i < len(requests) and requests[i].arrival_s <= now:
-                if padded(request_bytes(requests[i])) > self.budget:
-                    res.rejected += 1
-                else:
-                    queue.append(requests[i])
-                i += 1
-            # admission: FIFO while memory is free
-            batch, regions = [], []
-            while queue:
-                s = self.arena.alloc(padded(request_bytes(queue[0])))
-                if s is None:
-                    break
-                batch.append(queue.popleft())
-                regions.append(s)
-            if batch:
-                ev = self.issue(batch, regions)
-                inflight.append(_Inflight(batch, ev, regions))
-                res.iterations += 1
-                res.max_batch = max(res.max_batch, len(batch))
-                res.max_inflight = max(res.max_inflight, len(inflight))
-            elif not inflight and i < len(requests):
-                time.sleep(max(0.0, min(2e-4, requests[i].arrival_s - now)))
+            j = i
+            while j < len(requests) and requests[j].arrival_s <= now:
+                j += 1
+            if j > i:
+                for k, t in zip(range(i, j), self.submit(requests[i:j])):
+                    if t:
+                        owner[t] = requests[k]
+                    else:
+                        res.rejected += 1
+                i = j
+            done, pending = self.poll()
+            res.polls += 1
+            stamp = time.perf_counter() - t0
+            for t, _, nbytes in done:
+                res.latencies_s.append(stamp - owner.pop(t).arrival_s)
+                res.bytes += nbytes
+            if i == len(requests) and pending == 0:
+                break
+            if not done and pending == 0 and i < len(requests):
+                time.sleep(max(0.0, min(2e-4, requests[i].arrival_s - stamp)))
         res.wall_s = time.perf_counter() - t0
         return res
+
+
+def copy_to_host(ptr: int, nbytes: int, dtype: str) -> np.ndarray:
+    """A synchronous device-to-host copy of a completed output."""
+    out = np.empty(nbytes // ITEMSIZE[dtype], dtype=NUMPY[dtype])
+    hip = R._hip()
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    R._hip_check(hip.hipMemcpy(out.ctypes.data, ptr, nbytes, 2))  # hipMemcpyDeviceToHost
+    return out

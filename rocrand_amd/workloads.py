@@ -80,6 +80,7 @@ class Request:
     n: int
     seed: int
     dims: int = 1
+    session: int = 0  # 0: a fresh stream; else the server session it continues
 
 
 def serving_mix(count: int, rate: float, seed: int = 11) -> list[Request]:

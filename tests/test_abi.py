@@ -41,6 +41,23 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, s)
 
 
+def test_library_exports_every_extension_symbol():
+    """The extension headers beside rocrand.h (batched, serving, tuning) declare entry
+    points with no reference counterpart; librocrand.so exports every one."""
+    out = subprocess.run(["nm", "-D", "--defined-only", R.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\sT\s(rocrand_\w+)", out))
+    inc = os.path.join(ROOT, "include")
+    seen = 0
+    for name in sorted(os.listdir(inc)):
+        if not name.endswith(".h") or name == "rocrand.h":
+            continue
+        text = open(os.path.join(inc, name)).read()
+        for s in sorted(set(re.findall(r"ROCRANDAPI\s+rocrand_status\s+(rocrand_\w+)\s*\(", text))):
+            assert s in exported, f"{name}: {s}"
+            seen += 1
+    assert seen == 13
+
+
 def test_version():
     assert R.get_version() == 300200
     assert R.load().rocrand_get_version(None) == OUT_OF_RANGE
