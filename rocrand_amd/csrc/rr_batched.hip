@@ -705,9 +705,25 @@ std::tuple<int, int, int> tune_key(const group& g)
     return std::make_tuple(g.philox ? 1 : 0, (int)g.kind, lg);
 }
 
-// Chooses each group's grid size: the cached choice for its shape, else the fastest of a
-// few candidates timed on the plan's own outputs (the values written are the same for
-// every grid size).
+// Two timing events, destroyed on every return path.
+struct event_pair
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    bool       create() { return hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess; }
+    ~event_pair()
+    {
+        if(a)
+            (void)hipEventDestroy(a);
+        if(b)
+            (void)hipEventDestroy(b);
+    }
+};
+
+// Chooses each group's grid size: the cached choice for its shape, else the candidate
+// with the lowest median of TUNE_RUNS timed launches (after one untimed launch) on the
+// plan's own outputs (the values written are the same for every grid size).
+constexpr int TUNE_RUNS = 3;
+
 rocrand_status tune(rocrand_batch_plan_st& p)
 {
     static const uint32_t candidates[] = {1024, 2048, 4096, 8192, 16384};
@@ -725,36 +741,36 @@ rocrand_status tune(rocrand_batch_plan_st& p)
                 continue;
             }
         }
-        hipEvent_t a, b;
-        if(hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+        event_pair ev;
+        if(!ev.create())
             return ROCRAND_STATUS_INTERNAL_ERROR;
         float    best_ms = 1e30f;
         uint32_t best    = g.blocks;
         for(uint32_t c : candidates)
         {
             const uint32_t blocks = std::min<uint32_t>(c, (uint32_t)g.tiles.size());
-            // one untimed run, then one timed
-            for(int rep = 0; rep < 2; rep++)
+            float          ms[TUNE_RUNS];
+            for(int rep = -1; rep < TUNE_RUNS; rep++)
             {
-                if(rep == 1 && hipEventRecord(a, p.stream) != hipSuccess)
+                if(rep >= 0 && hipEventRecord(ev.a, p.stream) != hipSuccess)
                     return ROCRAND_STATUS_INTERNAL_ERROR;
-                if(g.fn(g.d_calls, g.d_tiles, (uint32_t)g.tiles.size(), blocks, g.lds_bytes, p.tables, p.stream) != hipSuccess)
+                if(g.fn(g.d_calls, g.d_tiles, (uint32_t)g.tiles.size(), blocks, g.lds_bytes, p.tables, p.stream)
+                   != hipSuccess)
                     return ROCRAND_STATUS_LAUNCH_FAILURE;
+                if(rep >= 0
+                   && (hipEventRecord(ev.b, p.stream) != hipSuccess || hipEventSynchronize(ev.b) != hipSuccess
+                       || hipEventElapsedTime(&ms[rep], ev.a, ev.b) != hipSuccess))
+                    return ROCRAND_STATUS_INTERNAL_ERROR;
             }
-            float ms = 0.f;
-            if(hipEventRecord(b, p.stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess
-               || hipEventElapsedTime(&ms, a, b) != hipSuccess)
-                return ROCRAND_STATUS_INTERNAL_ERROR;
-            if(ms < best_ms)
+            std::sort(ms, ms + TUNE_RUNS);
+            if(ms[TUNE_RUNS / 2] < best_ms)
             {
-                best_ms = ms;
+                best_ms = ms[TUNE_RUNS / 2];
                 best    = c;
             }
             if(blocks == g.tiles.size())
                 break; // larger grids change nothing
         }
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
         g.blocks = best;
         std::lock_guard<std::mutex> lock(g_tune_mtx);
         g_tuned[key] = best;

@@ -1,15 +1,17 @@
 """Autotuner of the DYNAMIC ordering's launch grids on the current GPU.
 
-    python -m rocrand_amd.tune_dynamic [--n N] [--reps R] [--inc PATH] [--report PATH]
+    python -m rocrand_amd.tune_dynamic [--n N] [--reps R] [--calls C] [--inc PATH] [--report PATH]
 
 For every engine with a DYNAMIC grid it times candidate grids through the library itself:
-each candidate is installed with rocrand_amd_set_dynamic_grid, a fresh generator is set to
-ROCRAND_ORDERING_PSEUDO_DYNAMIC (so it initialises its engines for that grid), and every
-output kind of the bulk path is generated at N values per call; per kind the median of R
-timed repetitions (10 calls each, HIP events on the generator's stream) after a warm-up
-gives GB/s.  A candidate's score is the geometric mean over the kinds; the best score wins,
-except that the DEFAULT grid is kept unless a candidate beats it by more than 2% (so that
-DYNAMIC output stays DEFAULT output where the grid makes no measurable difference).
+each candidate is installed with rocrand_amd_set_dynamic_grid and a fresh generator set to
+ROCRAND_ORDERING_PSEUDO_DYNAMIC is initialised (taking that grid for its lifetime).  After
+0.5 s of calls that bring the GPU's clocks to their working level, R rounds each time C
+calls of every output kind of the bulk path (N values per call, HIP events on the
+generators' stream) for every candidate in turn, so that clock drift falls on all
+candidates alike; per (candidate, kind) the median over the rounds gives GB/s.  A
+candidate's score is the geometric mean over the kinds; the best score wins, except that
+the DEFAULT grid is kept unless a candidate beats it by more than 2% (so that DYNAMIC
+output stays DEFAULT output where the grid makes no measurable difference).
 
 Candidates: the interleaved engines (MRG32k3a, MRG31k3p, XORWOW, LFSR113) run one engine
 per lane, so their engine count is their parallelism: 256 x {256, 512, 1024, 2048, 4096}.
@@ -60,31 +62,53 @@ def kinds(torch):
             ("poisson_10", torch.uint32, lambda g, b: g.poisson(b, 10.0))]
 
 
-def measure(R, torch, stream, rng, grid, n, reps, bufs):
-    R.set_dynamic_grid(rng, *grid)
-    g = R.PRNG(rng, seed=12345)
+def timed(torch, stream, fn, calls):
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(calls):
+        fn()
+    end.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / calls
+
+
+def measure(R, torch, stream, rng, grids, n, reps, calls, bufs):
+    """{grid: {kind: GB/s}}: medians over `reps` rounds, candidates interleaved per round."""
+    gens = {}
+    for grid in grids:
+        R.set_dynamic_grid(rng, *grid)
+        g = R.PRNG(rng, seed=12345)
+        g.stream = stream
+        g.set_ordering(R.ORDERING_PSEUDO_DYNAMIC)
+        g.initialize()
+        gens[grid] = g
+    R.set_dynamic_grid(rng, 0, 0)
+    ks = kinds(torch)
+    for _, dt, _ in ks:
+        bufs.setdefault(dt, torch.empty(n, dtype=dt, device="cuda"))
+    times = {grid: {k: [] for k, _, _ in ks} for grid in grids}
+    for _ in range(reps):
+        for grid, g in gens.items():
+            for kname, dt, call in ks:
+                b = bufs[dt]
+                call(g, b)  # the kind's first call of the round is not timed
+                times[grid][kname].append(timed(torch, stream, lambda: call(g, b), calls))
+    out = {}
+    for grid in grids:
+        out[grid] = {k: n * bufs[dt].element_size() / (statistics.median(times[grid][k]) * 1e-3) / 1e9
+                     for k, dt, _ in ks}
+    return out
+
+
+def settle(R, torch, stream, n, bufs, seconds=0.5):
+    g = R.PRNG(R.PHILOX4_32_10, seed=1)
     g.stream = stream
-    g.set_ordering(R.ORDERING_PSEUDO_DYNAMIC)
-    g.initialize()
-    row = {}
-    for kname, dt, call in kinds(torch):
-        b = bufs.setdefault(dt, torch.empty(n, dtype=dt, device="cuda"))
-        for _ in range(5):
-            call(g, b)
+    b = bufs.setdefault(torch.float32, torch.empty(n, dtype=torch.float32, device="cuda"))
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(20):
+            g.uniform(b)
         torch.cuda.synchronize()
-        times = []
-        for _ in range(reps):
-            start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            start.record(stream)
-            for _ in range(10):
-                call(g, b)
-            end.record(stream)
-            torch.cuda.synchronize()
-            times.append(start.elapsed_time(end) / 10)
-        ms = statistics.median(times)
-        row[kname] = n * b.element_size() / (ms * 1e-3) / 1e9
-    del g
-    return row
 
 
 def geomean(row):
@@ -94,7 +118,8 @@ def geomean(row):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=1 << 26)
-    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--calls", type=int, default=20)
     ap.add_argument("--inc", default=DEFAULT_INC)
     ap.add_argument("--report", default=None)
     args = ap.parse_args(argv)
@@ -107,16 +132,17 @@ def main(argv=None):
     stream = torch.cuda.Stream()
     bufs = {}
     results, chosen = {}, {}
+    settle(R, torch, stream, args.n, bufs)
     for name in COUNTER + INTERLEAVED:
         rng = getattr(R, name)
         cands, default = candidates(name)
+        rows = measure(R, torch, stream, rng, cands, args.n, args.reps, args.calls, bufs)
         scores = {}
         for grid in cands:
-            row = measure(R, torch, stream, rng, grid, args.n, args.reps, bufs)
+            row = rows[grid]
             scores[grid] = (geomean(row), row)
             print(f"# {name} {grid[0]}x{grid[1]}: geomean {scores[grid][0]:.1f} GB/s "
                   + " ".join(f"{k}={v:.0f}" for k, v in row.items()), flush=True)
-        R.set_dynamic_grid(rng, 0, 0)
         best = max(scores, key=lambda gr: scores[gr][0])
         if scores[best][0] <= 1.02 * scores[default][0]:
             best = default
@@ -125,8 +151,9 @@ def main(argv=None):
 
     stamp = time.strftime("%Y-%m-%d")
     lines = [f"// DYNAMIC-ordering launch grids for {arch} (threads, blocks per engine), written by",
-             f"// rocrand_amd/tune_dynamic.py on {stamp} (--n {args.n}, {args.reps} repetitions of 10 calls",
-             "// per output kind; geometric mean of GB/s over the kinds; DEFAULT kept unless beaten by 2%)."]
+             f"// rocrand_amd/tune_dynamic.py on {stamp} (--n {args.n}, median of {args.reps} interleaved rounds of",
+             f"// {args.calls} calls per output kind; geometric mean of GB/s over the kinds; DEFAULT kept",
+             "// unless beaten by 2%)."]
     for name in COUNTER + INTERLEAVED:
         t, b = chosen[name]
         lines.append(f"{{{ENUM[name]}, {t}, {b}}}, // {results[name][(t, b)][0]:.0f} GB/s")
@@ -136,8 +163,10 @@ def main(argv=None):
 
     if args.report:
         rep = [f"# DYNAMIC-ordering grid sweep on {arch}", "",
-               f"Command: `python -m rocrand_amd.tune_dynamic --n {args.n} --reps {args.reps}` ({stamp}).",
-               "GB/s per output kind (median of the repetitions), geometric mean, and the grid chosen.", ""]
+               f"Command: `python -m rocrand_amd.tune_dynamic --n {args.n} --reps {args.reps} --calls {args.calls}` "
+               f"({stamp}).",
+               "GB/s per output kind (median of the interleaved rounds), geometric mean, and the grid chosen "
+               "(DEFAULT kept unless beaten by more than 2%).", ""]
         for name in COUNTER + INTERLEAVED:
             knames = list(next(iter(results[name].values()))[1])
             rep.append(f"## {name} (chosen {chosen[name][0]} x {chosen[name][1]})")
