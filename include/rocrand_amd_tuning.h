@@ -20,7 +20,8 @@ extern "C" {
 #endif
 
 /* Overrides the DYNAMIC grid of rng_type in this process (threads * blocks a multiple of
- * 256, at most 2^22; threads = blocks = 0 removes the override).  Takes effect when a
+ * 256, at most 2^22, and a power of two for Philox and ThreeFry; threads = blocks = 0
+ * removes the override).  Takes effect when a
  * generator next initialises (after rocrand_set_ordering, set_seed or set_offset).
  * TYPE_ERROR for engines without a grid (MT19937, the quasi engines), OUT_OF_RANGE for
  * sizes the kernels do not take. */

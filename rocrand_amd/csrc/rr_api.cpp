@@ -505,6 +505,8 @@ public:
             if(st != ROCRAND_STATUS_SUCCESS)
                 return st;
             m_stride = threads * blocks;
+            if((m_stride & (m_stride - 1)) != 0) // the kernels mask vec indices with it
+                return ROCRAND_STATUS_INTERNAL_ERROR;
         }
         // engine(seed, subsequence 0, offset): the stream starts at word `offset`.
         m_pos         = offset;

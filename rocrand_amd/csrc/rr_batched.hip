@@ -91,8 +91,9 @@ __global__ __launch_bounds__(256) void batch_counter_kernel(const counter_call* 
     const uint64_t* staged = nullptr;
     for(uint32_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x)
     {
-        const batch_tile tl = tiles[ti];
-        counter_call     c  = calls[tl.stream];
+        const batch_tile tl    = tiles[ti];
+        counter_call     c     = calls[tl.stream];
+        const uint64_t   rmask = (uint64_t)calls[tl.stream].ref_stride - 1;
         stage_tile_table<D>(c.params, staged);
         const size_t end = tl.first + tl.count;
         if(fast_ok<K>(c))
@@ -110,10 +111,10 @@ __global__ __launch_bounds__(256) void batch_counter_kernel(const counter_call* 
         else
         {
             for(size_t v = tl.first + threadIdx.x; v < end; v += 256)
-                generic_vec<K, D>(c, v);
+                generic_vec<K, D>(c, v, rmask);
         }
         if(tl.first == 0 && threadIdx.x == 0)
-            head_tail<K, D>(c);
+            head_tail<K, D>(c, rmask);
     }
 }
 

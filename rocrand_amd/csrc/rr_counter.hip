@@ -21,9 +21,9 @@ __global__ __launch_bounds__(256) void counter_kernel(counter_call c)
 {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for(size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < c.vec_n; v += stride)
-        generic_vec<K, D>(c, v);
+        generic_vec<K, D>(c, v, (uint64_t)c.ref_stride - 1);
     if(blockIdx.x == 0 && threadIdx.x == 0)
-        head_tail<K, D>(c);
+        head_tail<K, D>(c, (uint64_t)c.ref_stride - 1);
 }
 
 template<ctr_kind K, class D, bool SUB>
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void counter_fast_kernel(counter_call c)
     else if(wave * TILE < c.vec_n)
         fast_tile<K, D, SUB>(c, wave * TILE, c.vec_n);
     if(blockIdx.x == 0 && threadIdx.x == 0)
-        head_tail<K, D>(c);
+        head_tail<K, D>(c, (uint64_t)c.ref_stride - 1);
 }
 
 template<ctr_kind K, class D>

@@ -16,6 +16,8 @@ namespace rr
 namespace ctr_dev
 {
 
+
+
 template<class W, int DIM>
 __device__ __forceinline__ void interleave(const W* a, const W* b, uint32_t sub, W* o)
 {
@@ -105,8 +107,10 @@ __device__ __forceinline__ void store_vec(void* dst, const uint32_t* u)
 }
 
 // Head and tail of a call (at most FULL-1 outputs each), written by one thread.
+// stride_mask = c.ref_stride - 1 (the stride is a power of two), passed by the caller: a
+// read of the field here keeps the batched kernels' per-tile copy of the call in scratch.
 template<ctr_kind K, class D>
-__device__ void head_tail(const counter_call& c)
+__device__ void head_tail(const counter_call& c, uint64_t stride_mask)
 {
     using T                    = ctr_traits<K>;
     using W                    = typename T::word;
@@ -139,7 +143,7 @@ __device__ void head_tail(const counter_call& c)
         if(c.tail > 0)
         {
             // Tail: the thread whose next vec index is vec_n continues from that vec's position.
-            const ctr_pos& b = (c.quirk && (c.vec_n % c.ref_stride) == 0) ? c.vec_base0 : c.vec_base;
+            const ctr_pos  b = (c.quirk && (c.vec_n & stride_mask) == 0) ? c.vec_base0 : c.vec_base;
             seq_reader<K>  r(b, c.vec_n, c.key);
             for(int s = 0; s < OPT; s++)
             {
@@ -162,7 +166,7 @@ __device__ void head_tail(const counter_call& c)
 
 // One vec of a call, any position (two engine blocks when the stream starts mid-block).
 template<ctr_kind K, class D>
-__device__ __forceinline__ void generic_vec(const counter_call& c, size_t v)
+__device__ __forceinline__ void generic_vec(const counter_call& c, size_t v, uint64_t stride_mask)
 {
     using T                    = ctr_traits<K>;
     using W                    = typename T::word;
@@ -177,7 +181,7 @@ __device__ __forceinline__ void generic_vec(const counter_call& c, size_t v)
     static_assert(BYTES == 8 || BYTES == 16 || BYTES == 32, "vec size");
 
     out_t* const   vec_data = static_cast<out_t*>(c.data) + c.mis;
-    const ctr_pos& b        = (c.quirk && (v % c.ref_stride) == 0) ? c.vec_base0 : c.vec_base;
+    const ctr_pos  b        = (c.quirk && (v & stride_mask) == 0) ? c.vec_base0 : c.vec_base;
     W              ctr[DIM];
 #pragma unroll
     for(int i = 0; i < DIM; i++)

@@ -128,6 +128,13 @@ rocrand_status ROCRANDAPI rocrand_amd_set_dynamic_grid(rocrand_rng_type rng_type
     const unsigned long long size = (unsigned long long)threads * blocks;
     if(size != 0 && !rr::valid_size(size))
         return ROCRAND_STATUS_OUT_OF_RANGE;
+    // the counter engines' kernels test vec indices against the stride with a mask
+    const bool counter = rng_type == ROCRAND_RNG_PSEUDO_PHILOX4_32_10 || rng_type == ROCRAND_RNG_PSEUDO_THREEFRY2_32_20
+                         || rng_type == ROCRAND_RNG_PSEUDO_THREEFRY2_64_20
+                         || rng_type == ROCRAND_RNG_PSEUDO_THREEFRY4_32_20
+                         || rng_type == ROCRAND_RNG_PSEUDO_THREEFRY4_64_20;
+    if(counter && (size & (size - 1)) != 0)
+        return ROCRAND_STATUS_OUT_OF_RANGE;
     // MTGP32 runs one 256-thread block per generator and its parameter sets: its DYNAMIC
     // grid is the DEFAULT one (fewer generators would only lower its parallelism)
     if(rng_type == ROCRAND_RNG_PSEUDO_MTGP32 && size != 0 && (threads != 256 || blocks != 512))

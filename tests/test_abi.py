@@ -180,3 +180,18 @@ def test_sobol_host_decided_status_codes(t):
             assert L.rocrand_generate_long_long(g, None, 9) == TYPE_ERROR
     finally:
         assert L.rocrand_destroy_generator(g) == SUCCESS
+
+
+def test_dynamic_grid_override_status_codes():
+    """rocrand_amd_set_dynamic_grid decides on the host (include/rocrand_amd_tuning.h)."""
+    L = R.load()
+    assert L.rocrand_amd_set_dynamic_grid(R.MT19937, 256, 256) == TYPE_ERROR
+    assert L.rocrand_amd_set_dynamic_grid(R.MRG32K3A, 100, 3) == OUT_OF_RANGE
+    assert L.rocrand_amd_set_dynamic_grid(R.MRG32K3A, 256, 1 << 15) == OUT_OF_RANGE  # above 2^22
+    assert L.rocrand_amd_set_dynamic_grid(R.MTGP32, 256, 256) == OUT_OF_RANGE
+    assert L.rocrand_amd_set_dynamic_grid(R.PHILOX4_32_10, 256, 768) == OUT_OF_RANGE
+    assert L.rocrand_amd_set_dynamic_grid(R.THREEFRY4_64_20, 256, 3) == OUT_OF_RANGE
+    assert L.rocrand_amd_set_dynamic_grid(R.MRG32K3A, 256, 768) == SUCCESS  # engine counts need not be
+    assert L.rocrand_amd_set_dynamic_grid(R.MRG32K3A, 0, 0) == SUCCESS
+    assert L.rocrand_amd_set_dynamic_grid(R.PHILOX4_32_10, 256, 2048) == SUCCESS
+    assert L.rocrand_amd_set_dynamic_grid(R.PHILOX4_32_10, 0, 0) == SUCCESS

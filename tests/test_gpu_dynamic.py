@@ -85,4 +85,6 @@ def test_dynamic_grid_errors():
         R.set_dynamic_grid(R.MT19937, 256, 256)  # no grid for MT19937
     with pytest.raises(R.RocRandError):
         R.set_dynamic_grid(R.MTGP32, 256, 256)  # MTGP32 keeps 256 x 512
+    with pytest.raises(R.RocRandError):
+        R.set_dynamic_grid(R.PHILOX4_32_10, 256, 768)  # counter strides are powers of two
     assert R.get_dynamic_grid(R.MTGP32) == (256, 512)
