@@ -361,6 +361,16 @@ __host__ __device__ __forceinline__ uint32_t mrg32_fold_final(uint64_t x, uint32
     return w < v ? w : v;
 }
 
+// p1 - p2 + (p1 <= p2 ? m : 0) (rocrand_mrg32k3a.h:190-192, rocrand_mrg31k3p.h next()) from
+// one subtraction's borrow: e = p2 - p1 wraps exactly when p1 > p2, and then 0 - e is
+// p1 - p2; otherwise m - e is p1 - p2 + m (m itself when p1 == p2).
+__host__ __device__ __forceinline__ uint32_t mrg_combine(uint32_t p1, uint32_t p2, uint32_t m)
+{
+    uint32_t   e;
+    const bool gt = __builtin_sub_overflow(p2, p1, &e);
+    return (gt ? 0u : m) - e;
+}
+
 __host__ __device__ __forceinline__ uint32_t mrg32k3a_next(mrg_state& s)
 {
     // t1 < (a12 + a13n) m1 < 2214308 * 2^32, so one fold by 2^32 = 209 (mod m1) leaves
@@ -380,7 +390,7 @@ __host__ __device__ __forceinline__ uint32_t mrg32k3a_next(mrg_state& s)
     s.g2[0]           = s.g2[1];
     s.g2[1]           = s.g2[2];
     s.g2[2]           = p2;
-    return (p1 - p2) + (p1 <= p2 ? (uint32_t)MRG32_M1 : 0u);
+    return mrg_combine(p1, p2, (uint32_t)MRG32_M1);
 }
 
 // Seeded state words {g1[0..2], g2[0..2]} (rocrand_mrg32k3a.h:103-120, :318-363): the
@@ -451,7 +461,7 @@ __host__ __device__ __forceinline__ uint32_t mrg31k3p_next(mrg_state& s)
     s.g2[2] = s.g2[1];
     s.g2[1] = s.g2[0];
     s.g2[0] = tmp;
-    return s.g1[0] - s.g2[0] + (s.g1[0] <= s.g2[0] ? MRG31_M1 : 0u);
+    return mrg_combine(s.g1[0], s.g2[0], MRG31_M1);
 }
 
 // x mod (2^K - C) for any 64-bit x: three folds by 2^K = C leave x < 2m for the four
