@@ -209,15 +209,15 @@ def batched_bench(R, torch, count=4096, seed=7, launches=10):
             "graph_GBps": round(_gbs(total_bytes, graph_ms), 1), "outputs_equal_plain": same}
 
 
-def serving_bench(torch, rates=(1000, 10000, 50000), seconds=1.0, budget_gb=32):
+def serving_bench(torch, rates=(1000, 10000, 50000), seconds=1.0, budget_gb=32, streams=4):
     """Open-loop Poisson arrivals at each rate for `seconds` (rocrand_amd.workloads.serving_mix:
     sizes log-uniform in [256, 2^26], 50% Philox / 30% MRG32k3a / 20% Sobol64 x16 dims,
     f32/f64/u32) through the request server (include/rocrand_serving.h) with a 32 GB HBM
-    budget and two streams."""
+    budget and `streams` streams."""
     from rocrand_amd.serving import Server
     from rocrand_amd.workloads import serving_mix
 
-    srv = Server(int(budget_gb * 1e9), streams=2)
+    srv = Server(int(budget_gb * 1e9), streams=streams)
     # warm-up: kernels, tables
     srv.serve(serving_mix(50, 1e6, seed=1))
     out = []
@@ -230,7 +230,7 @@ def serving_bench(torch, rates=(1000, 10000, 50000), seconds=1.0, budget_gb=32):
                     "wall_s": round(res.wall_s, 3)})
         print(f"# serving: {out[-1]}", file=sys.stderr, flush=True)
     srv.close()
-    return {"budget_GB": budget_gb, "workload_seed": 11, "runs": out}
+    return {"budget_GB": budget_gb, "streams": streams, "workload_seed": 11, "runs": out}
 
 
 def _exact_box_muller_f32(u, v):
@@ -373,6 +373,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=1 << 28)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-streams", type=int, default=4)
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain 2^28-float device copy (counter calibration)")
     ap.add_argument("--only", metavar="ENGINE:KIND",
@@ -426,7 +427,7 @@ def main():
     batched = None if args.no_batched else batched_bench(R, torch)
     if batched:
         print(f"# batched: {batched}", file=sys.stderr, flush=True)
-    serving = None if args.no_serving else serving_bench(torch)
+    serving = None if args.no_serving else serving_bench(torch, streams=args.serving_streams)
 
     line = {
         "metric": "output GB/s & % of HBM write peak per generator; batched-vs-plain speedup",

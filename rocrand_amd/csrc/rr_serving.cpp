@@ -2,7 +2,9 @@
 //
 // The loop's state lives here: the HBM arena (first fit over one allocation, freed ranges
 // coalesced), the FIFO queue, the in-flight batches with their completion events, and the
-// sessions' generators.  A batch goes to one of the server's streams in turn; its
+// sessions' generators.  Each stream holds at most one batch in flight, and a poll
+// issues everything admissible as one batch on a free stream, so that requests arriving
+// while the streams are busy are coalesced into the next batch.  A batch's
 // Philox4x32-10 requests (fresh or session) and fresh MRG32k3a requests are one
 // rocrand_generate_batched call, every MRG32k3a session or Sobol64 request one plain call of
 // its generator.  Ordering: a request's output depends only on its own call sequence, so
@@ -49,7 +51,8 @@ struct issued
 
 struct batch
 {
-    hipEvent_t          done = nullptr;
+    hipEvent_t          done   = nullptr;
+    unsigned            stream = 0;
     std::vector<issued> reqs;
 };
 
@@ -88,7 +91,7 @@ struct rocrand_server_st
     char*                                         mem    = nullptr;
     size_t                                        budget = 0;
     std::vector<hipStream_t>                      streams;
-    unsigned                                      turn = 0;
+    std::vector<char>                             busy; // a batch of this stream is in flight
     int                                           keep = 0;
     rr::arena                                     free_mem{0};
     std::deque<queued>                            queue;
@@ -126,14 +129,24 @@ struct rocrand_server_st
             }
             for(const issued& q : it->reqs)
                 done.push_back(q);
+            busy[it->stream] = 0;
             spare.push_back(it->done);
             it = inflight.erase(it);
         }
     }
 
-    rocrand_status issue(const std::vector<queued>& reqs, const std::vector<size_t>& starts)
+    // A stream with no batch in flight, or streams.size().
+    unsigned free_stream() const
     {
-        hipStream_t                       s = streams[turn++ % streams.size()];
+        unsigned i = 0;
+        while(i < streams.size() && busy[i])
+            i++;
+        return i;
+    }
+
+    rocrand_status issue(const std::vector<queued>& reqs, const std::vector<size_t>& starts, unsigned si)
+    {
+        hipStream_t                       s = streams[si];
         std::vector<rocrand_batch_stream> bs;
         for(size_t i = 0; i < reqs.size(); i++)
         {
@@ -206,6 +219,8 @@ struct rocrand_server_st
         }
         for(size_t i = 0; i < reqs.size(); i++)
             b.reqs.push_back(issued{reqs[i].ticket, starts[i], reqs[i].bytes});
+        b.stream  = si;
+        busy[si]  = 1;
         inflight.push_back(std::move(b));
         return ROCRAND_STATUS_SUCCESS;
     }
@@ -266,6 +281,7 @@ rocrand_status ROCRANDAPI rocrand_server_create(rocrand_server* server, size_t b
             }
             s->streams.push_back(q);
         }
+        s->busy.assign(streams, 0);
     }
     catch(const std::bad_alloc&)
     {
@@ -388,10 +404,12 @@ rocrand_status ROCRANDAPI rocrand_server_poll(rocrand_server server, rocrand_ser
             server->done.pop_front();
         }
         *completed = k;
-        // admission: first come first served while the outputs fit
+        // admission, when a stream is free: first come first served while the outputs fit;
+        // while every stream is busy, arrivals wait and then go out together
         std::vector<queued> reqs;
         std::vector<size_t> starts;
-        while(!server->queue.empty())
+        const unsigned      si = server->free_stream();
+        while(si < server->streams.size() && !server->queue.empty())
         {
             size_t start = 0;
             if(!server->free_mem.alloc(slot(server->queue.front().bytes), start))
@@ -402,7 +420,7 @@ rocrand_status ROCRANDAPI rocrand_server_poll(rocrand_server server, rocrand_ser
         }
         if(!reqs.empty())
         {
-            const rocrand_status st = server->issue(reqs, starts);
+            const rocrand_status st = server->issue(reqs, starts, si);
             if(st != ROCRAND_STATUS_SUCCESS)
             {
                 // the batch is dropped: let whatever it launched finish, then free its memory
