@@ -163,8 +163,8 @@ def main(argv=None):
 
     if args.report:
         rep = [f"# DYNAMIC-ordering grid sweep on {arch}", "",
-               f"Command: `python -m rocrand_amd.tune_dynamic --n {args.n} --reps {args.reps} --calls {args.calls}` "
-               f"({stamp}).",
+               f"Command: `python -m rocrand_amd.tune_dynamic --n {args.n} --reps {args.reps} --calls {args.calls} "
+               f"--inc {args.inc} --report {args.report}` ({stamp}).",
                "GB/s per output kind (median of the interleaved rounds), geometric mean, and the grid chosen "
                "(DEFAULT kept unless beaten by more than 2%).", ""]
         for name in COUNTER + INTERLEAVED:
