@@ -22,14 +22,16 @@ namespace
 {
 
 // rocrand_discrete.h:109-165: binary search of the CDF; the draw is x * 2^-32
-// (32-bit values) or x * 2^-64 (64-bit values).
-__device__ __forceinline__ unsigned int discrete_cdf(double x, const alias_table& t)
+// (32-bit values) or x * 2^-64 (64-bit values).  The table is the block's copy in
+// dynamic LDS (sobol_kernel stages it), so every probe is an LDS read.
+__device__ __forceinline__ unsigned int discrete_cdf_lds(double x, const alias_table& t)
 {
+    extern __shared__ double lds_cdf[];
     unsigned int lo = 0, hi = t.size - 1;
     do
     {
         const unsigned int c = (lo + hi) / 2;
-        if(x > t.cdf[c])
+        if(x > lds_cdf[c])
             lo = c + 1;
         else
             hi = c;
@@ -100,7 +102,7 @@ __device__ __forceinline__ typename sob_out<K>::type sob_apply(V x, const dist_p
     {
         if(p.huge)
             return (uint32_t)round(__dadd_rn(__dmul_rn(p.sqrt_lambda, sob_normal_f64(x)), p.lambda));
-        return discrete_cdf(cdf_draw(x), p.pt);
+        return discrete_cdf_lds(cdf_draw(x), p.pt);
     }
 }
 
@@ -123,6 +125,20 @@ __global__ __launch_bounds__(256) void sobol_kernel(const sobol_call c)
     const V*       vec = static_cast<const V*>(c.vectors) + (size_t)dim * B;
     const V        sc  = c.scramble ? static_cast<const V*>(c.scramble)[dim] : V(0);
     T*             out = static_cast<T*>(c.data) + (size_t)dim * c.size;
+
+    // Poisson: the block searches its copy of the CDF in dynamic LDS (at most 4096
+    // doubles), so the binary search's dependent loads are LDS round trips; every thread
+    // copies before any leaves
+    if constexpr(K == out_kind::poisson)
+    {
+        extern __shared__ double lds_cdf[];
+        if(!c.params.huge)
+        {
+            for(uint32_t i = threadIdx.x; i < c.params.pt.size; i += blockDim.x)
+                lds_cdf[i] = c.params.pt.cdf[i];
+            __syncthreads();
+        }
+    }
 
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6);
@@ -165,8 +181,9 @@ __global__ __launch_bounds__(256) void sobol_kernel(const sobol_call c)
 template<class V>
 hipError_t dispatch(out_kind o, const sobol_call& c, dim3 grid, hipStream_t s)
 {
+    const size_t lds = o == out_kind::poisson && !c.params.huge ? c.params.pt.size * sizeof(double) : 0;
 #define RR_CASE(K)                                                                            \
-    case K: hipLaunchKernelGGL((sobol_kernel<V, K>), grid, dim3(256), 0, s, c); break;
+    case K: hipLaunchKernelGGL((sobol_kernel<V, K>), grid, dim3(256), lds, s, c); break;
     switch(o)
     {
         RR_CASE(out_kind::u8)
