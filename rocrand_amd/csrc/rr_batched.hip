@@ -7,11 +7,12 @@
 //     the per-call kernels do (rr_counter_dev.hpp), so outputs match the plain path bit
 //     for bit; the stream's head and tail go to its first tile;
 //   * MRG32k3a: a tile is up to 256 of a stream's engines.  A lane initialises its engine
-//     (the stream's seeded state advanced by offset / S + [engine < offset mod S] steps --
-//     done once per stream on the host -- then jumped by the engine index in
-//     subsequences, as the per-call init kernel does) and then writes that engine's vecs
-//     v = id, id + S, ... (rr_interleaved_dev.hpp).  Only the engines a stream uses are set
-//     up: min(S, vec_n + 1) instead of S per call.
+//     (the stream's seeded state advanced by offset / S + [engine < offset mod S] steps,
+//     jumped by the engine index in subsequences, as the per-call init kernel does) and
+//     then writes that engine's vecs v = id, id + S, ... (rr_interleaved_dev.hpp).  The
+//     tile's first engine state is computed on the host, so a lane applies one radix-table
+//     entry (its offset within the tile) rather than two.  Only the engines a stream uses
+//     are set up: min(S, vec_n + 1) instead of S per call.
 // The plan records the group launches in a HIP graph once; a launch replays it.  The
 // grid size of each group is chosen by timing a few candidates on the first plan of a
 // given shape (kept in a process-wide cache).
@@ -121,6 +122,7 @@ __global__ __launch_bounds__(256) void batch_counter_kernel(const counter_call* 
 template<inter_kind K, class D>
 __global__ __launch_bounds__(256) void batch_inter_kernel(const inter_call* calls,
                                                           const batch_tile* tiles,
+                                                          const uint32_t*   tile_states,
                                                           uint32_t          ntiles,
                                                           const device_tables t)
 {
@@ -138,10 +140,22 @@ __global__ __launch_bounds__(256) void batch_inter_kernel(const inter_call* call
         stage_tile_table<D>(c.params, staged);
         if(threadIdx.x >= tl.count)
             continue;
-        const uint32_t eid = (id + c.start) % S;
-        ieng<K>           g;
-        g.set(c.base[eid < c.start ? 1 : 0]);
-        g.sub(t, eid);
+        // engine (id + start) mod S; the host gave the state of the tile's first engine
+        // (first + start, or first + start - S when the whole tile has wrapped), so a lane
+        // is X^j from it, j = its lane index; in the tile that wraps, the wrapped lanes are
+        // X^(j - w) from base[1] (engine 0 with the extra step)
+        const uint32_t e0 = (uint32_t)tl.first + c.start;
+        const uint32_t w  = e0 < S ? S - e0 : 0u;
+        const bool     wr = e0 < S && threadIdx.x >= w;
+        uint32_t       w6[6];
+#pragma unroll
+        for(int q = 0; q < 6; q++)
+            w6[q] = wr ? c.base[1][q] : tile_states[(size_t)ti * 6 + q];
+        ieng<K> g;
+        g.set(w6);
+        const uint32_t j = wr ? threadIdx.x - w : threadIdx.x;
+        if(j != 0u)
+            g.sub_entry(t, j);
         out_t* const data     = static_cast<out_t*>(c.data);
         out_t* const vec_data = data + c.mis;
         alignas(16) out_t o[OUT];
@@ -199,6 +213,22 @@ __global__ __launch_bounds__(256) void batch_inter_kernel(const inter_call* call
     }
 }
 
+// The device's radix jump (rr_interleaved_dev.hpp mrg_sub_jump) on the host, MRG32k3a.
+void host_entry_apply(const uint32_t* entry, uint32_t x[6])
+{
+    mat3_vec_fold<32, 209>(entry, x);
+    mat3_vec_fold<32, 22853>(entry + 9, x + 3);
+}
+void host_sub_jump(const uint32_t* radix, uint32_t e, uint32_t x[6])
+{
+    const uint32_t lo = e & (uint32_t)(MRG_RADIX_LO - 1);
+    const uint32_t hi = (e >> MRG_RADIX_LO_BITS) & (uint32_t)(MRG_RADIX_HI - 1);
+    if(lo != 0u)
+        host_entry_apply(radix + lo * MRG_RADIX_WORDS, x);
+    if(hi != 0u)
+        host_entry_apply(radix + (MRG_RADIX_LO + hi) * MRG_RADIX_WORDS, x);
+}
+
 out_kind kind_of(rocrand_batch_kind k)
 {
     switch(k)
@@ -214,13 +244,15 @@ out_kind kind_of(rocrand_batch_kind k)
     }
 }
 
-// `lds` = bytes of dynamic LDS for the group's largest Poisson table (0 otherwise).
-using launcher = hipError_t (*)(const void* calls, const batch_tile* tiles, uint32_t ntiles, uint32_t blocks,
-                                uint32_t lds, const device_tables& t, hipStream_t s);
+// `lds` = bytes of dynamic LDS for the group's largest Poisson table (0 otherwise);
+// `tile_states` = MRG32k3a groups' per-tile engine states (6 words a tile), else null.
+using launcher = hipError_t (*)(const void* calls, const batch_tile* tiles, const uint32_t* tile_states,
+                                uint32_t ntiles, uint32_t blocks, uint32_t lds, const device_tables& t,
+                                hipStream_t s);
 
 template<ctr_kind K, class D>
-hipError_t launch_counter_group(const void* calls, const batch_tile* tiles, uint32_t ntiles, uint32_t blocks,
-                                uint32_t lds, const device_tables&, hipStream_t s)
+hipError_t launch_counter_group(const void* calls, const batch_tile* tiles, const uint32_t*, uint32_t ntiles,
+                                uint32_t blocks, uint32_t lds, const device_tables&, hipStream_t s)
 {
     hipLaunchKernelGGL((batch_counter_kernel<K, D>), dim3(blocks), dim3(256), lds, s,
                        static_cast<const counter_call*>(calls), tiles, ntiles);
@@ -228,11 +260,11 @@ hipError_t launch_counter_group(const void* calls, const batch_tile* tiles, uint
 }
 
 template<inter_kind K, class D>
-hipError_t launch_inter_group(const void* calls, const batch_tile* tiles, uint32_t ntiles, uint32_t blocks,
-                              uint32_t lds, const device_tables& t, hipStream_t s)
+hipError_t launch_inter_group(const void* calls, const batch_tile* tiles, const uint32_t* tile_states,
+                              uint32_t ntiles, uint32_t blocks, uint32_t lds, const device_tables& t, hipStream_t s)
 {
     hipLaunchKernelGGL((batch_inter_kernel<K, D>), dim3(blocks), dim3(256), lds, s,
-                       static_cast<const inter_call*>(calls), tiles, ntiles, t);
+                       static_cast<const inter_call*>(calls), tiles, tile_states, ntiles, t);
     return hipGetLastError();
 }
 
@@ -267,18 +299,52 @@ struct group
     std::vector<counter_call> ccalls;
     std::vector<inter_call>   icalls;
     std::vector<batch_tile>   tiles;
+    std::vector<uint32_t>     tile_states; // MRG32k3a: first engine state of each tile
     size_t                    bytes_out = 0;
     size_t                    calls_at  = 0; // offsets in the descriptor image
     size_t                    tiles_at  = 0;
+    size_t                    states_at = 0;
     void*                     d_calls   = nullptr;
     batch_tile*               d_tiles   = nullptr;
+    const uint32_t*           d_states  = nullptr;
     uint32_t                  blocks    = 2048;
+    uint32_t                  tile_vecs = CTR_TILE_VECS; // Philox: vecs per tile
     uint32_t                  lds_bytes = 0; // largest Poisson table of the group, packed
 };
 
-// Grid sizes chosen per (engine, kind, log2 of the group's output bytes).
+// Philox tiles of a group: each stream's vecs in runs of `tile` (at least one tile per
+// stream: its first tile also writes head and tail).
+std::vector<batch_tile> counter_tiles(const group& g, uint32_t tile)
+{
+    std::vector<batch_tile> t;
+    for(uint32_t si = 0; si < (uint32_t)g.ccalls.size(); si++)
+    {
+        const size_t vec_n = g.ccalls[si].vec_n;
+        size_t       v     = 0;
+        do
+        {
+            const uint32_t cnt = (uint32_t)std::min<size_t>(tile, vec_n - v);
+            t.push_back(batch_tile{si, cnt, v});
+            v += cnt;
+        } while(v < vec_n);
+    }
+    return t;
+}
+
+// Schedules chosen per (engine, kind, log2 of the group's output bytes): grid size and,
+// for Philox, vecs per tile (the items each thread takes from a tile: tile / 256).
+struct schedule
+{
+    uint32_t blocks, tile_vecs;
+};
 std::mutex                                    g_tune_mtx;
-std::map<std::tuple<int, int, int>, uint32_t> g_tuned;
+std::map<std::tuple<int, int, int>, schedule> g_tuned;
+
+std::tuple<int, int, int> tune_key(const group& g)
+{
+    const int lg = g.bytes_out ? (int)std::log2((double)g.bytes_out) : 0;
+    return std::make_tuple(g.philox ? 1 : 0, (int)g.kind, lg);
+}
 
 // A batch on the host: groups, the distinct Poisson tables, and the layout of the one
 // descriptor image (tables, then each group's calls and tiles) that goes to the device.
@@ -311,6 +377,8 @@ struct batch_layout
             at = align(at + (g.philox ? g.ccalls.size() * sizeof(counter_call) : g.icalls.size() * sizeof(inter_call)));
             g.tiles_at = at;
             at         = align(at + g.tiles.size() * sizeof(batch_tile));
+            g.states_at = at;
+            at          = align(at + g.tile_states.size() * sizeof(uint32_t));
         }
         bytes = std::max<size_t>(at, 256);
     }
@@ -348,8 +416,10 @@ struct batch_layout
                 memcpy(host + g.calls_at, g.icalls.data(), g.icalls.size() * sizeof(inter_call));
             }
             memcpy(host + g.tiles_at, g.tiles.data(), g.tiles.size() * sizeof(batch_tile));
-            g.d_calls = dev + g.calls_at;
-            g.d_tiles = reinterpret_cast<batch_tile*>(dev + g.tiles_at);
+            memcpy(host + g.states_at, g.tile_states.data(), g.tile_states.size() * sizeof(uint32_t));
+            g.d_calls  = dev + g.calls_at;
+            g.d_tiles  = reinterpret_cast<batch_tile*>(dev + g.tiles_at);
+            g.d_states = g.tile_states.empty() ? nullptr : reinterpret_cast<const uint32_t*>(dev + g.states_at);
         }
     }
 };
@@ -361,7 +431,7 @@ hipError_t run_groups(const std::vector<group>& groups, const device_tables& tab
         if(g.tiles.empty())
             continue;
         const uint32_t blocks = std::min<uint32_t>(g.blocks, (uint32_t)g.tiles.size());
-        hipError_t     e      = g.fn(g.d_calls, g.d_tiles, (uint32_t)g.tiles.size(), blocks, g.lds_bytes, tables, s);
+        hipError_t     e      = g.fn(g.d_calls, g.d_tiles, g.d_states, (uint32_t)g.tiles.size(), blocks, g.lds_bytes, tables, s);
         if(e != hipSuccess)
             return e;
     }
@@ -412,7 +482,7 @@ hipError_t run_groups_forked(const std::vector<group>& groups, const device_tabl
             continue;
         const uint32_t blocks = std::min<uint32_t>(g.blocks, (uint32_t)g.tiles.size());
         if((e = hipStreamWaitEvent(fj.side[i], fj.fork, 0)) != hipSuccess
-           || (e = g.fn(g.d_calls, g.d_tiles, (uint32_t)g.tiles.size(), blocks, g.lds_bytes, tables, fj.side[i])) != hipSuccess
+           || (e = g.fn(g.d_calls, g.d_tiles, g.d_states, (uint32_t)g.tiles.size(), blocks, g.lds_bytes, tables, fj.side[i])) != hipSuccess
            || (e = hipEventRecord(fj.done[i], fj.side[i])) != hipSuccess
            || (e = hipStreamWaitEvent(cap, fj.done[i], 0)) != hipSuccess)
             break;
@@ -649,14 +719,6 @@ rocrand_status build(batch_layout& L, const rocrand_batch_stream* streams, size_
                 return st;
             c.params = prm;
             g.ccalls.push_back(c);
-            // at least one tile per stream: its first tile also writes head and tail
-            size_t v = 0;
-            do
-            {
-                const uint32_t cnt = (uint32_t)std::min<size_t>(CTR_TILE_VECS, c.vec_n - v);
-                g.tiles.push_back(batch_tile{si, cnt, v});
-                v += cnt;
-            } while(v < c.vec_n);
         }
         else
         {
@@ -691,19 +753,54 @@ rocrand_status build(batch_layout& L, const rocrand_batch_stream* streams, size_
             g.icalls.push_back(c);
             const bool   extra = s.out_w > 1 && (c.head > 0 || c.tail > 0);
             const size_t lanes = std::min<size_t>(INTERLEAVED_ENGINES, c.vec_n + (extra ? 1 : 0));
+            // Tile states: X^(first + start) base[0] while the tile's first engine has not
+            // wrapped, X^(first + start - S) base[1] after; consecutive tiles of a region
+            // differ by X^256 (radix entry 256).
+            const uint32_t* radix = host_mrg_radix(32);
+            uint32_t        x[6];
+            bool            have = false, wrapped = false;
             for(size_t e = 0; e < lanes; e += MRG_TILE)
+            {
                 g.tiles.push_back(batch_tile{si, (uint32_t)std::min<size_t>(MRG_TILE, lanes - e), e});
+                const uint64_t e0 = e + c.start;
+                if(!have || (!wrapped && e0 >= INTERLEAVED_ENGINES))
+                {
+                    wrapped = e0 >= INTERLEAVED_ENGINES;
+                    std::copy(c.base[wrapped ? 1 : 0], c.base[wrapped ? 1 : 0] + 6, x);
+                    host_sub_jump(radix, (uint32_t)(wrapped ? e0 - INTERLEAVED_ENGINES : e0), x);
+                    have = true;
+                }
+                else
+                    host_entry_apply(radix + 256 * MRG_RADIX_WORDS, x);
+                g.tile_states.insert(g.tile_states.end(), x, x + 6);
+            }
         }
         g.bytes_out += b.n * (size_t)s.esz;
     }
+    // Philox tiles once the group's size (and so its tuned tile size, if any) is known
+    for(auto& g : L.groups)
+        if(g.philox)
+        {
+            {
+                std::lock_guard<std::mutex> lock(g_tune_mtx);
+                auto                        it = g_tuned.find(tune_key(g));
+                if(it != g_tuned.end())
+                {
+                    g.blocks    = it->second.blocks;
+                    g.tile_vecs = it->second.tile_vecs;
+                }
+            }
+            g.tiles = counter_tiles(g, g.tile_vecs);
+        }
+        else
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mtx);
+            auto                        it = g_tuned.find(tune_key(g));
+            if(it != g_tuned.end())
+                g.blocks = it->second.blocks;
+        }
     L.place();
     return ROCRAND_STATUS_SUCCESS;
-}
-
-std::tuple<int, int, int> tune_key(const group& g)
-{
-    const int lg = g.bytes_out ? (int)std::log2((double)g.bytes_out) : 0;
-    return std::make_tuple(g.philox ? 1 : 0, (int)g.kind, lg);
 }
 
 // Two timing events, destroyed on every return path.
@@ -720,14 +817,30 @@ struct event_pair
     }
 };
 
-// Chooses each group's grid size: the cached choice for its shape, else the candidate
-// with the lowest median of TUNE_RUNS timed launches (after one untimed launch) on the
-// plan's own outputs (the values written are the same for every grid size).
+// Device memory released on every return path.
+struct device_block
+{
+    void* p = nullptr;
+    ~device_block()
+    {
+        if(p)
+            (void)hipFree(p);
+    }
+};
+
+// Chooses each group's schedule: the cached choice for its shape, else the candidate with
+// the lowest median of TUNE_RUNS timed launches (after one untimed launch) on the plan's
+// own outputs (the values written are the same for every schedule).  Candidates: grid
+// sizes, and for Philox groups tiles of 1024, 4096 and 16384 vecs (4, 16 or 64 vecs per
+// thread), each tiling uploaded to a scratch block for its timings.  Sets `relayout` when
+// a group's chosen tile size differs from the one the plan was laid out with.
 constexpr int TUNE_RUNS = 3;
 
-rocrand_status tune(rocrand_batch_plan_st& p)
+rocrand_status tune(rocrand_batch_plan_st& p, bool& relayout)
 {
-    static const uint32_t candidates[] = {1024, 2048, 4096, 8192, 16384};
+    static const uint32_t grids[]       = {1024, 2048, 4096, 8192, 16384};
+    static const uint32_t ctr_tiles[]   = {1024, CTR_TILE_VECS, 16384};
+    relayout                            = false;
     for(auto& g : p.groups)
     {
         if(g.tiles.empty())
@@ -738,7 +851,8 @@ rocrand_status tune(rocrand_batch_plan_st& p)
             auto                        it = g_tuned.find(key);
             if(it != g_tuned.end())
             {
-                g.blocks = it->second;
+                g.blocks = it->second.blocks;
+                relayout = relayout || (g.philox && it->second.tile_vecs != g.tile_vecs);
                 continue;
             }
         }
@@ -746,33 +860,54 @@ rocrand_status tune(rocrand_batch_plan_st& p)
         if(!ev.create())
             return ROCRAND_STATUS_INTERNAL_ERROR;
         float    best_ms = 1e30f;
-        uint32_t best    = g.blocks;
-        for(uint32_t c : candidates)
+        schedule best{g.blocks, g.tile_vecs};
+        const size_t ntile_cands = g.philox ? sizeof(ctr_tiles) / sizeof(ctr_tiles[0]) : 1;
+        for(size_t tc = 0; tc < ntile_cands; tc++)
         {
-            const uint32_t blocks = std::min<uint32_t>(c, (uint32_t)g.tiles.size());
-            float          ms[TUNE_RUNS];
-            for(int rep = -1; rep < TUNE_RUNS; rep++)
+            const uint32_t    tile   = g.philox ? ctr_tiles[tc] : g.tile_vecs;
+            const batch_tile* tiles  = g.d_tiles;
+            uint32_t          ntiles = (uint32_t)g.tiles.size();
+            device_block      tmp;
+            if(g.philox && tile != g.tile_vecs)
             {
-                if(rep >= 0 && hipEventRecord(ev.a, p.stream) != hipSuccess)
+                const std::vector<batch_tile> tt = counter_tiles(g, tile);
+                if(hipMalloc(&tmp.p, tt.size() * sizeof(batch_tile)) != hipSuccess)
+                    return ROCRAND_STATUS_ALLOCATION_FAILED;
+                if(hipMemcpy(tmp.p, tt.data(), tt.size() * sizeof(batch_tile), hipMemcpyHostToDevice) != hipSuccess)
                     return ROCRAND_STATUS_INTERNAL_ERROR;
-                if(g.fn(g.d_calls, g.d_tiles, (uint32_t)g.tiles.size(), blocks, g.lds_bytes, p.tables, p.stream)
-                   != hipSuccess)
-                    return ROCRAND_STATUS_LAUNCH_FAILURE;
-                if(rep >= 0
-                   && (hipEventRecord(ev.b, p.stream) != hipSuccess || hipEventSynchronize(ev.b) != hipSuccess
-                       || hipEventElapsedTime(&ms[rep], ev.a, ev.b) != hipSuccess))
-                    return ROCRAND_STATUS_INTERNAL_ERROR;
+                tiles  = static_cast<const batch_tile*>(tmp.p);
+                ntiles = (uint32_t)tt.size();
             }
-            std::sort(ms, ms + TUNE_RUNS);
-            if(ms[TUNE_RUNS / 2] < best_ms)
+            for(uint32_t c : grids)
             {
-                best_ms = ms[TUNE_RUNS / 2];
-                best    = c;
+                const uint32_t blocks = std::min<uint32_t>(c, ntiles);
+                float          ms[TUNE_RUNS];
+                for(int rep = -1; rep < TUNE_RUNS; rep++)
+                {
+                    if(rep >= 0 && hipEventRecord(ev.a, p.stream) != hipSuccess)
+                        return ROCRAND_STATUS_INTERNAL_ERROR;
+                    if(g.fn(g.d_calls, tiles, g.d_states, ntiles, blocks, g.lds_bytes, p.tables, p.stream) != hipSuccess)
+                        return ROCRAND_STATUS_LAUNCH_FAILURE;
+                    if(rep >= 0
+                       && (hipEventRecord(ev.b, p.stream) != hipSuccess || hipEventSynchronize(ev.b) != hipSuccess
+                           || hipEventElapsedTime(&ms[rep], ev.a, ev.b) != hipSuccess))
+                        return ROCRAND_STATUS_INTERNAL_ERROR;
+                }
+                std::sort(ms, ms + TUNE_RUNS);
+                if(ms[TUNE_RUNS / 2] < best_ms)
+                {
+                    best_ms = ms[TUNE_RUNS / 2];
+                    best    = schedule{c, tile};
+                }
+                if(blocks == ntiles)
+                    break; // larger grids change nothing
             }
-            if(blocks == g.tiles.size())
-                break; // larger grids change nothing
+            // the scratch tiling's kernels must finish before its block is freed
+            if(tmp.p && hipStreamSynchronize(p.stream) != hipSuccess)
+                return ROCRAND_STATUS_INTERNAL_ERROR;
         }
-        g.blocks = best;
+        g.blocks = best.blocks;
+        relayout = relayout || best.tile_vecs != g.tile_vecs;
         std::lock_guard<std::mutex> lock(g_tune_mtx);
         g_tuned[key] = best;
     }
@@ -796,25 +931,38 @@ rocrand_status ROCRANDAPI rocrand_batch_plan_create(rocrand_batch_plan*         
     {
         p         = new rocrand_batch_plan_st();
         p->stream = stream;
-        batch_layout   L;
-        rocrand_status st = build(L, streams, count);
-        if(st == ROCRAND_STATUS_SUCCESS)
-        {
+        // lay the batch out (with the cached schedules) and upload its descriptor image
+        auto lay_out = [&]() {
+            batch_layout   L;
+            rocrand_status r = build(L, streams, count);
+            if(r != ROCRAND_STATUS_SUCCESS)
+                return r;
             std::vector<char> image(L.bytes);
             if(hipMalloc(reinterpret_cast<void**>(&p->d_image), L.bytes) != hipSuccess)
-                st = ROCRAND_STATUS_ALLOCATION_FAILED;
-            else
             {
-                L.pack(image.data(), p->d_image);
-                if(hipMemcpy(p->d_image, image.data(), L.bytes, hipMemcpyHostToDevice) != hipSuccess)
-                    st = ROCRAND_STATUS_INTERNAL_ERROR;
+                p->d_image = nullptr;
+                return ROCRAND_STATUS_ALLOCATION_FAILED;
             }
+            L.pack(image.data(), p->d_image);
             p->groups = std::move(L.groups);
-        }
+            return hipMemcpy(p->d_image, image.data(), L.bytes, hipMemcpyHostToDevice) == hipSuccess
+                       ? ROCRAND_STATUS_SUCCESS
+                       : ROCRAND_STATUS_INTERNAL_ERROR;
+        };
+        rocrand_status st = lay_out();
         if(st == ROCRAND_STATUS_SUCCESS && get_device_tables(p->tables) != hipSuccess)
             st = ROCRAND_STATUS_INTERNAL_ERROR;
+        bool relayout = false;
         if(st == ROCRAND_STATUS_SUCCESS)
-            st = tune(*p);
+            st = tune(*p, relayout);
+        if(st == ROCRAND_STATUS_SUCCESS && relayout)
+        {
+            // a tuned tile size differs from the layout's: lay out again (every timing run
+            // has completed: each ended with an event synchronisation)
+            (void)hipFree(p->d_image);
+            p->d_image = nullptr;
+            st         = lay_out();
+        }
         if(st == ROCRAND_STATUS_SUCCESS && hipEventCreateWithFlags(&p->done, hipEventDisableTiming) != hipSuccess)
             st = ROCRAND_STATUS_INTERNAL_ERROR;
         if(st == ROCRAND_STATUS_SUCCESS)
@@ -908,13 +1056,6 @@ rocrand_status ROCRANDAPI rocrand_generate_batched(const rocrand_batch_stream* s
         int           device = 0;
         if(get_device_tables(tables) != hipSuccess || hipGetDevice(&device) != hipSuccess)
             return ROCRAND_STATUS_INTERNAL_ERROR;
-        for(auto& g : L.groups)
-        {
-            std::lock_guard<std::mutex> lock(g_tune_mtx);
-            auto                        it = g_tuned.find(tune_key(g));
-            if(it != g_tuned.end())
-                g.blocks = it->second;
-        }
         // Descriptors go through a pooled pinned buffer and a pooled device block, both
         // released to the pool when the event recorded after the kernels has fired: the
         // call returns without waiting for the device.
@@ -966,13 +1107,6 @@ rocrand_status ROCRANDAPI rocrand_generate_batched_noalloc(const rocrand_batch_s
         char*         dev = nullptr;
         if(get_device_tables(tables) != hipSuccess || (dev = scratch_address()) == nullptr)
             return ROCRAND_STATUS_INTERNAL_ERROR;
-        for(auto& g : L.groups)
-        {
-            std::lock_guard<std::mutex> lock(g_tune_mtx);
-            auto                        it = g_tuned.find(tune_key(g));
-            if(it != g_tuned.end())
-                g.blocks = it->second;
-        }
         // Descriptors are packed in host memory and copied into the scratch array on the
         // stream; a copy from pageable memory has read the host buffer when it returns.
         std::vector<char> image(L.bytes);

@@ -95,6 +95,11 @@ struct ieng<inter_kind::mrg32k3a>
                 mat3_vec_fold<32, 22853>(tab + (i * 2 + 1) * 9, s.g2);
             }
     }
+    // X^j for j < 512: one radix-table entry
+    __device__ void sub_entry(const device_tables& t, uint32_t j)
+    {
+        mrg_entry_apply<32, 209, 22853>(t.mrg32_subr + j * MRG_RADIX_WORDS, s);
+    }
     __device__ void sub(const device_tables& t, uint32_t e)
     {
         mrg_sub_jump<32, 209, 22853>(t.mrg32_subr, e, s);
@@ -123,6 +128,10 @@ struct ieng<inter_kind::mrg31k3p> : ieng<inter_kind::mrg32k3a>
                 mat3_vec_fold<31, 1>(tab + (i * 2 + 0) * 9, s.g1);
                 mat3_vec_fold<31, 21069>(tab + (i * 2 + 1) * 9, s.g2);
             }
+    }
+    __device__ void sub_entry(const device_tables& t, uint32_t j)
+    {
+        mrg_entry_apply<31, 1, 21069>(t.mrg31_subr + j * MRG_RADIX_WORDS, s);
     }
     __device__ void sub(const device_tables& t, uint32_t e)
     {

@@ -131,6 +131,9 @@ hipError_t get_device_tables(device_tables& t);
 
 // Host copies of the same tables (used by host-side seeding helpers and tests).
 const uint32_t* host_mrg_table(int which_engine /*32 or 31*/, int sub /*0 step,1 subsequence*/);
+// Host copy of the subsequence radix table (MRG_RADIX_LO + MRG_RADIX_HI entries of
+// MRG_RADIX_WORDS words: X^e for e < 512, then X^(512 h) for h < 256, X = A^(2^76) or A^(2^72)).
+const uint32_t* host_mrg_radix(int which_engine /*32 or 31*/);
 const uint32_t* host_gf2_table(int xorwow /*1 xorwow, 0 lfsr*/, int seq);
 
 } // namespace rr

@@ -267,6 +267,12 @@ const uint32_t* host_mrg_table(int which_engine, int sub)
     return sub ? h.mrg31_sub.data() : h.mrg31_step.data();
 }
 
+const uint32_t* host_mrg_radix(int which_engine)
+{
+    const host_tables& h = host();
+    return which_engine == 32 ? h.mrg32_subr.data() : h.mrg31_subr.data();
+}
+
 const uint32_t* host_gf2_table(int xorwow, int seq)
 {
     const host_tables& h = host();
