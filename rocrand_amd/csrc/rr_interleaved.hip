@@ -268,16 +268,27 @@ hipError_t launch_gen(const interleaved_call& c, hipStream_t s)
     const uint32_t S = c.S;
     if constexpr(K == inter_kind::xorwow || K == inter_kind::lfsr113)
     {
-        // XORWOW / LFSR113 write cheap outputs at the store rate with S lanes, and the
-        // ALU-bound transforms (Box-Muller, exp) find enough independent work in the
-        // unrolled loop; Poisson's dependent table loads need more wavefronts: chunks.
-        if constexpr(dist_kind<D>::value == out_kind::poisson)
+        // XORWOW / LFSR113 write cheap outputs at the store rate with S lanes (2 wavefronts
+        // per SIMD).  Poisson's dependent table loads and the longest transforms need more
+        // wavefronts: chunks of C a power of 4, at most JMAX per engine.  Measured at 2^28
+        // outputs (A/B of builds on one box): J <= 4 takes XORWOW log-normal f32 1.34 ->
+        // 2.03 TB/s, normal f64 1.90 -> 1.98, LFSR113 log-normal f32 1.33 -> 1.82, normal
+        // f32 2.70 -> 2.90, normal f64 1.67 -> 1.77, while XORWOW normal f32 lost 2% (kept
+        // on S lanes) and J <= 16 was slower than S lanes for every transform.
+        constexpr out_kind ok   = dist_kind<D>::value;
+        constexpr uint64_t JMAX = ok == out_kind::poisson ? 16
+                                  : (ok == out_kind::lognormal_f32 || ok == out_kind::lognormal_f64
+                                     || ok == out_kind::normal_f64
+                                     || (ok == out_kind::normal_f32 && K == inter_kind::lfsr113))
+                                      ? 4
+                                      : 0;
+        if constexpr(JMAX > 0)
         {
             const uint64_t kmax = (c.vec_n + S - 1) / S;
             if(kmax >= 64)
             {
                 uint64_t C = 16;
-                while((kmax + C - 1) / C > 16)
+                while((kmax + C - 1) / C > JMAX)
                     C *= 4;
                 const uint32_t J = (uint32_t)((kmax + C - 1) / C);
                 device_tables  t;

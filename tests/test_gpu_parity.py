@@ -193,8 +193,10 @@ LARGE_CASES = {
     # MRG: every kind chunks above 64 vec-steps per engine
     "mrg": [("u32", (1 << 24) + 3, 1), ("uniform_f32", 1 << 25, 0), ("normal_f32", (1 << 24) + 1, 1),
             ("uniform_f64", (1 << 23) + 5, 1)],
-    # XORWOW / LFSR113: Poisson chunks (C a power of 4); the others stay on S lanes
-    "gf2": [("poisson", (1 << 23) + 7, 0), ("poisson", (1 << 26) + 5, 3), ("normal_f32", (1 << 24) + 1, 1)],
+    # XORWOW / LFSR113: Poisson and the longest transforms chunk (C a power of 4) at 64 or
+    # more vec-steps per engine; XORWOW's normal f32 stays on S lanes
+    "gf2": [("poisson", (1 << 23) + 7, 0), ("poisson", (1 << 26) + 5, 3), ("normal_f32", (1 << 24) + 1, 1),
+            ("lognormal_f32", (1 << 24) + 3, 1), ("normal_f64", (1 << 24) + 1, 1), ("lognormal_f64", 1 << 25, 2)],
 }
 
 
