@@ -4,7 +4,8 @@ output kind of that configuration (u32, uniform f32/f64, normal f32/f64 at (0, 1
 log-normal f32 at (0, 1), Poisson at lambda 0.5, 10 and 1000), against the oracle; and
 single calls of the full 2^28 values for Philox4x32-10 and MRG32k3a, uniform and normal
 f32, checked at sampled windows (Philox: 64 windows of 1024 values plus both ends, the
-window rule pinned on the CPU in test_serving_host.py) or whole (MRG32k3a).
+window rule pinned on the CPU in test_serving_host.py) or whole (MRG32k3a); and the
+state-heavy and quasi-random configuration below.
 """
 import numpy as np
 import pytest
@@ -97,3 +98,51 @@ def test_mrg32k3a_full_size_whole(kind):
     st, want = O.OracleGen(R.MRG32K3A, seed=SEED).generate(kind, FULL, 0x10000000, 0.0, 1.0)
     assert st == 0
     check(kind, got, want, (0.0, 1.0))
+
+
+# The state-heavy and quasi-random engines' configuration: MT19937 and MTGP32 at 2^28
+# values (whole outputs against the oracle), Sobol32/64 and their scrambled variants with
+# 1, 64 and 1000 dimensions of 2^20 points at offsets 0 and 1000 (windows of every
+# checked dimension; the dimension-major layout is value d * 2^20 + j for point j).
+@pytest.mark.parametrize("kind", ["u32", "uniform_f32"])
+@pytest.mark.parametrize("name,rng", [("mt19937", R.MT19937), ("mtgp32", R.MTGP32)])
+def test_state_heavy_full_size(name, rng, kind):
+    g = R.PRNG(rng, seed=SEED)
+    dt = torch.uint32 if kind == "u32" else torch.float32
+    buf = torch.empty(FULL, dtype=dt, device="cuda")
+    (g.generate if kind == "u32" else g.uniform)(buf)
+    torch.cuda.synchronize()
+    got = buf.cpu().numpy()
+    del buf
+    print(f"case {name} {kind} 2^28", flush=True)
+    st, want = O.OracleGen(rng, seed=SEED).generate(kind, FULL, 0x10000000)
+    assert st == 0
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+SOBOL = {"sobol32": R.SOBOL32, "scrambled_sobol32": R.SCRAMBLED_SOBOL32, "sobol64": R.SOBOL64,
+         "scrambled_sobol64": R.SCRAMBLED_SOBOL64}
+NPD = 1 << 20
+
+
+@pytest.mark.parametrize("offset", [0, 1000])
+@pytest.mark.parametrize("dims", [1, 64, 1000])
+@pytest.mark.parametrize("name", list(SOBOL))
+def test_sobol_benchmark_configuration(name, dims, offset):
+    rng = SOBOL[name]
+    g = R.QRNG(rng, ndim=dims, offset=offset)
+    buf = torch.empty(dims * NPD, dtype=torch.uint32, device="cuda")
+    g.generate(buf)
+    torch.cuda.synchronize()
+    k = 1024
+    checked_dims = sorted({0, 1 % dims, dims // 2, dims - 1})
+    print(f"case {name} dims={dims} offset={offset}", flush=True)
+    for j0 in (0, NPD // 2 + 1000, NPD - k):
+        og = O.OracleGen(rng, dims=dims, offset=offset + j0)
+        st, want = og.generate("u32", dims * k, 0x10000000)
+        assert st == 0
+        want = want.reshape(dims, k)
+        for d in checked_dims:
+            got = buf[d * NPD + j0:d * NPD + j0 + k].cpu().numpy()
+            assert np.array_equal(got, want[d]), f"dimension {d} points {j0}.."
+    del buf

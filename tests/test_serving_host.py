@@ -215,3 +215,21 @@ def test_philox_offset_window_rule_for_full_size_calls(kind):
         st, w = O.OracleGen(R.PHILOX4_32_10, seed=12345, offset=base + i0).generate(kind, 1024, 0x10000000, 0.0, 1.0)
         assert st == 0
         check_exact(w, whole[i0:i0 + 1024])
+
+
+@pytest.mark.parametrize("rng", [R.SOBOL32, R.SCRAMBLED_SOBOL32, R.SOBOL64, R.SCRAMBLED_SOBOL64])
+def test_quasi_window_rule_every_variant(rng):
+    """The point-window rule the Sobol benchmark-configuration test uses: points
+    [j0, j0 + k) of every dimension are a fresh generator at offset + j0."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    dims, npd, off = 64, 4096, 1000
+    st, whole = O.OracleGen(rng, dims=dims, offset=off).generate("u32", dims * npd, 0x10000000)
+    assert st == 0
+    whole = whole.reshape(dims, npd)
+    for j0 in (0, 1234, npd - 1024):
+        st, w = O.OracleGen(rng, dims=dims, offset=off + j0).generate("u32", dims * 1024, 0x10000000)
+        assert st == 0
+        assert np.array_equal(w.reshape(dims, 1024), whole[:, j0:j0 + 1024]), j0
