@@ -67,8 +67,9 @@ ROCRANDAPI rocrand_status rocrand_server_create(rocrand_server* server, size_t b
 ROCRANDAPI rocrand_status rocrand_server_destroy(rocrand_server server);
 /* Queues `count` requests; tickets[i] receives request i's ticket (increasing from 1), or 0
  * when its output cannot fit the budget (rejected).  TYPE_ERROR for an unsupported engine
- * or a session request that changes its session's engine, seed or dimensions;
- * LENGTH_NOT_MULTIPLE for a Sobol64 length that is not a multiple of dims. */
+ * or a session request that changes its session's engine, seed or dimensions (also
+ * against an earlier request of the same call); LENGTH_NOT_MULTIPLE for a Sobol64 length
+ * that is not a multiple of dims.  A call that returns an error queues nothing. */
 ROCRANDAPI rocrand_status rocrand_server_submit(rocrand_server               server,
                                                 const rocrand_serve_request* requests,
                                                 size_t                       count,

@@ -63,7 +63,7 @@ struct session
     unsigned int       dims = 1;
     rocrand_generator  gen  = nullptr; // MRG32k3a, Sobol64
     unsigned long long pos  = 0;       // Philox4x32-10: words consumed
-    hipEvent_t         last = nullptr; // MRG32k3a: after its latest request
+    hipEvent_t         last = nullptr; // MRG32k3a, Sobol64: after its latest request
     hipStream_t        last_stream = nullptr;
 };
 
@@ -199,14 +199,13 @@ struct rocrand_server_st
                 if((st = rocrand_set_stream(ss.gen, s)) != ROCRAND_STATUS_SUCCESS
                    || (st = generate_kind(ss.gen, r.dtype, out, r.n)) != ROCRAND_STATUS_SUCCESS)
                     return st;
-                if(r.rng_type == ROCRAND_RNG_PSEUDO_MRG32K3A)
-                {
-                    if(ss.last == nullptr && (ss.last = event()) == nullptr)
-                        return ROCRAND_STATUS_INTERNAL_ERROR;
-                    if(hipEventRecord(ss.last, s) != hipSuccess)
-                        return ROCRAND_STATUS_LAUNCH_FAILURE;
-                    ss.last_stream = s;
-                }
+                // after the session's latest request: the next MRG32k3a request waits for
+                // it, and ending the session waits for it before destroying the generator
+                if(ss.last == nullptr && (ss.last = event()) == nullptr)
+                    return ROCRAND_STATUS_INTERNAL_ERROR;
+                if(hipEventRecord(ss.last, s) != hipSuccess)
+                    return ROCRAND_STATUS_LAUNCH_FAILURE;
+                ss.last_stream = s;
             }
         }
         batch b;
@@ -309,7 +308,9 @@ rocrand_status ROCRANDAPI rocrand_server_submit(rocrand_server server, const roc
         return ROCRAND_STATUS_OUT_OF_RANGE;
     try
     {
-        // validate everything first: a rejected call queues nothing
+        // validate everything first: a rejected call queues nothing (sessions a request of
+        // this call opens are checked against its later requests too)
+        std::unordered_map<unsigned long long, session> opened;
         for(size_t i = 0; i < count; i++)
         {
             const rocrand_serve_request& r = requests[i];
@@ -326,11 +327,55 @@ rocrand_status ROCRANDAPI rocrand_server_submit(rocrand_server server, const roc
             if(r.session != 0)
             {
                 auto it = server->sessions.find(r.session);
-                if(it != server->sessions.end()
-                   && (it->second.type != r.rng_type || it->second.seed != r.seed || it->second.dims != dims))
+                if(it == server->sessions.end())
+                {
+                    it = opened.find(r.session);
+                    if(it == opened.end())
+                    {
+                        session ss;
+                        ss.type = r.rng_type;
+                        ss.seed = r.seed;
+                        ss.dims = dims;
+                        it      = opened.emplace(r.session, ss).first;
+                    }
+                }
+                if(it->second.type != r.rng_type || it->second.seed != r.seed || it->second.dims != dims)
                     return ROCRAND_STATUS_TYPE_ERROR;
             }
         }
+        // then the generators of new sessions (a failure here queues nothing) ...
+        for(size_t i = 0; i < count; i++)
+        {
+            const rocrand_serve_request& r = requests[i];
+            if(r.session == 0 || slot(r.n * elem_size(r.dtype)) > server->budget
+               || server->sessions.find(r.session) != server->sessions.end())
+                continue;
+            session ss;
+            ss.type = r.rng_type;
+            ss.seed = r.seed;
+            ss.dims = r.rng_type == ROCRAND_RNG_QUASI_SOBOL64 ? r.dims : 1u;
+            if(r.rng_type != ROCRAND_RNG_PSEUDO_PHILOX4_32_10)
+            {
+                rocrand_status st = rocrand_create_generator(&ss.gen, r.rng_type);
+                if(st != ROCRAND_STATUS_SUCCESS)
+                    return st;
+                if(r.rng_type == ROCRAND_RNG_QUASI_SOBOL64)
+                {
+                    st = rocrand_set_quasi_random_generator_dimensions(ss.gen, ss.dims);
+                    if(st == ROCRAND_STATUS_SUCCESS)
+                        st = rocrand_set_offset(ss.gen, r.seed % (1ull << 20));
+                }
+                else
+                    st = rocrand_set_seed(ss.gen, r.seed);
+                if(st != ROCRAND_STATUS_SUCCESS)
+                {
+                    (void)rocrand_destroy_generator(ss.gen);
+                    return st;
+                }
+            }
+            server->sessions[r.session] = ss;
+        }
+        // ... then the requests
         for(size_t i = 0; i < count; i++)
         {
             const rocrand_serve_request& r     = requests[i];
@@ -339,33 +384,6 @@ rocrand_status ROCRANDAPI rocrand_server_submit(rocrand_server server, const roc
             {
                 tickets[i] = 0; // can never fit
                 continue;
-            }
-            if(r.session != 0 && server->sessions.find(r.session) == server->sessions.end())
-            {
-                session ss;
-                ss.type = r.rng_type;
-                ss.seed = r.seed;
-                ss.dims = r.rng_type == ROCRAND_RNG_QUASI_SOBOL64 ? r.dims : 1u;
-                rocrand_status st = ROCRAND_STATUS_SUCCESS;
-                if(r.rng_type != ROCRAND_RNG_PSEUDO_PHILOX4_32_10)
-                {
-                    if((st = rocrand_create_generator(&ss.gen, r.rng_type)) != ROCRAND_STATUS_SUCCESS)
-                        return st;
-                    if(r.rng_type == ROCRAND_RNG_QUASI_SOBOL64)
-                    {
-                        st = rocrand_set_quasi_random_generator_dimensions(ss.gen, ss.dims);
-                        if(st == ROCRAND_STATUS_SUCCESS)
-                            st = rocrand_set_offset(ss.gen, r.seed % (1ull << 20));
-                    }
-                    else
-                        st = rocrand_set_seed(ss.gen, r.seed);
-                    if(st != ROCRAND_STATUS_SUCCESS)
-                    {
-                        (void)rocrand_destroy_generator(ss.gen);
-                        return st;
-                    }
-                }
-                server->sessions[r.session] = ss;
             }
             tickets[i] = server->next_ticket++;
             server->queue.push_back(queued{tickets[i], r, bytes});

@@ -175,6 +175,10 @@ def test_status_codes_and_edge_requests():
     assert L.rocrand_server_submit(h, one(req(R.MRG32K3A, 1, 6, 8, 100)), 1, t) == TYPE_ERROR
     assert L.rocrand_server_submit(h, one(req(R.PHILOX4_32_10, 1, 5, 8, 100)), 1, t) == TYPE_ERROR
     assert L.rocrand_server_end_session(h, 8) == OUT_OF_RANGE  # its request is still queued
+    # a call whose requests open one session with two engines queues nothing
+    two = (S.ServeRequest * 2)(req(R.MRG32K3A, 1, 5, 9, 100), req(R.SOBOL64, 1, 5, 9, 16, 16))
+    t2 = (S.ctypes.c_ulonglong * 2)()
+    assert L.rocrand_server_submit(h, two, 2, t2) == TYPE_ERROR
     # an empty request completes with no bytes
     tickets = srv.submit([Request(0.0, R.PHILOX4_32_10, "f32", 0, 3)])
     seen = {}
