@@ -582,6 +582,10 @@ public:
         c.vec_base0 = to_pos(m_pos + C0);
         c.quirk      = (H != C0) ? 1 : 0;
         c.ref_stride = m_stride;
+        // the fast kernels align their 64-vec chunks to the output's 128-byte lines
+        const uintptr_t vaddr  = addr + (uintptr_t)c.mis * s.esz;
+        const unsigned  vbytes = full * s.esz;
+        c.lane_shift = 128u % vbytes == 0 ? (uint32_t)((vaddr / vbytes) % (128u / vbytes)) : 0u;
         c.data       = data;
         c.n          = n;
     }

@@ -99,14 +99,25 @@ __global__ __launch_bounds__(256) void batch_counter_kernel(const counter_call* 
         const size_t end = tl.first + tl.count;
         if(fast_ok<K>(c))
         {
-            const uint32_t T    = c.vec_base.sub ? 63u : 64u;
-            const size_t   wave = threadIdx.x >> 6;
-            for(size_t f = tl.first + wave * T; f < end; f += 4 * T)
+            // the tile's 64-vec chunks on the stream's line-aligned grid (a tile other than
+            // the stream's first starts on it; the first starts lane_shift vecs before 0)
+            const int64_t  lo   = (int64_t)tl.first, hi = (int64_t)end;
+            const int64_t  vb0  = lo - ((lo + c.lane_shift) & 63);
+            const uint32_t nch  = (uint32_t)((hi - vb0 + 63) / 64);
+            const uint32_t wave = threadIdx.x >> 6;
+            if(c.vec_base.sub)
             {
-                if(c.vec_base.sub)
-                    fast_tile<K, D, true>(c, f, end);
-                else
-                    fast_tile<K, D, false>(c, f, end);
+                // consecutive chunks per wavefront, so that each run's look-ahead block
+                // serves its next chunk
+                const uint32_t q  = (nch + 3) / 4;
+                const uint32_t m0 = wave * q;
+                if(m0 < nch)
+                    fast_run<K, D, true>(c, vb0 + 64 * (int64_t)m0, std::min(q, nch - m0), lo, hi);
+            }
+            else
+            {
+                for(uint32_t m = wave; m < nch; m += 4)
+                    fast_run<K, D, false>(c, vb0 + 64 * (int64_t)m, 1, lo, hi);
             }
         }
         else
@@ -312,8 +323,10 @@ struct group
     uint32_t                  lds_bytes = 0; // largest Poisson table of the group, packed
 };
 
-// Philox tiles of a group: each stream's vecs in runs of `tile` (at least one tile per
-// stream: its first tile also writes head and tail).
+// Philox tiles of a group: each stream's vecs in runs of `tile` whose boundaries fall on
+// the stream's line-aligned chunk grid (multiples of 64 less its lane_shift), so no chunk
+// straddles two tiles; at least one tile per stream (its first tile also writes head and
+// tail).
 std::vector<batch_tile> counter_tiles(const group& g, uint32_t tile)
 {
     std::vector<batch_tile> t;
@@ -321,11 +334,13 @@ std::vector<batch_tile> counter_tiles(const group& g, uint32_t tile)
     {
         const size_t vec_n = g.ccalls[si].vec_n;
         size_t       v     = 0;
+        size_t       bound = tile - g.ccalls[si].lane_shift; // lane_shift < 64 <= tile
         do
         {
-            const uint32_t cnt = (uint32_t)std::min<size_t>(tile, vec_n - v);
+            const uint32_t cnt = (uint32_t)(std::min(bound, vec_n) - v);
             t.push_back(batch_tile{si, cnt, v});
             v += cnt;
+            bound += tile;
         } while(v < vec_n);
     }
     return t;

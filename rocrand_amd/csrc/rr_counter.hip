@@ -26,11 +26,28 @@ __global__ __launch_bounds__(256) void counter_kernel(counter_call c)
         head_tail<K, D>(c, (uint64_t)c.ref_stride - 1);
 }
 
+// Work of one wavefront: chunks of 64 vecs on the output's line-aligned grid -- one
+// without a mid-block start (every wavefront owns one chunk, so the resident wavefronts
+// write one contiguous front), 8 with one (a run computes one block beyond its chunks for
+// its last lane) -- except that ThreeFry with a mid-block start takes 63-vec tiles.
+template<ctr_kind K, bool SUB>
+constexpr bool tile63()
+{
+    return SUB && K != ctr_kind::philox4x32;
+}
+template<ctr_kind K, bool SUB>
+constexpr uint32_t fast_chunks()
+{
+    return SUB ? 8u : 1u;
+}
+
 template<ctr_kind K, class D, bool SUB>
 __global__ __launch_bounds__(256) void counter_fast_kernel(counter_call c)
 {
-    constexpr uint32_t TILE = SUB ? 63u : 64u;
+    constexpr uint32_t CH   = fast_chunks<K, SUB>();
     const size_t       wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t      vb   = (int64_t)(wave * CH * 64) - (int64_t)c.lane_shift;
+
     if constexpr(dist_kind<D>::value == out_kind::poisson)
     {
         // the packed alias table in LDS (dynamic, size * 8 bytes): one LDS read per sample
@@ -42,11 +59,21 @@ __global__ __launch_bounds__(256) void counter_fast_kernel(counter_call c)
         __syncthreads();
         counter_call cl   = c;
         cl.params.pt.packed = lds_packed;
-        if(wave * TILE < c.vec_n)
-            fast_tile<K, D, SUB>(cl, wave * TILE, c.vec_n);
+        if constexpr(tile63<K, SUB>())
+        {
+            if(wave * 63 < c.vec_n)
+                fast_tile63<K, D>(cl, wave * 63, c.vec_n);
+        }
+        else if(vb < (int64_t)c.vec_n)
+            fast_run<K, D, SUB>(cl, vb, CH, 0, (int64_t)c.vec_n);
     }
-    else if(wave * TILE < c.vec_n)
-        fast_tile<K, D, SUB>(c, wave * TILE, c.vec_n);
+    else if constexpr(tile63<K, SUB>())
+    {
+        if(wave * 63 < c.vec_n)
+            fast_tile63<K, D>(c, wave * 63, c.vec_n);
+    }
+    else if(vb < (int64_t)c.vec_n)
+        fast_run<K, D, SUB>(c, vb, CH, 0, (int64_t)c.vec_n);
     if(blockIdx.x == 0 && threadIdx.x == 0)
         head_tail<K, D>(c, (uint64_t)c.ref_stride - 1);
 }
@@ -61,9 +88,10 @@ hipError_t launch_one(const counter_call& c, hipStream_t s)
         // 64 bits would carry within the call (4-word engines only).
         if(fast_ok<K>(c))
         {
-            const uint32_t tile   = c.vec_base.sub ? 63u : 64u;
-            const size_t   tiles  = (c.vec_n + tile - 1) / tile; // one wavefront per tile
-            size_t         blocks = (tiles + 3) / 4;
+            const bool   t63    = c.vec_base.sub && K != ctr_kind::philox4x32;
+            const size_t per    = t63 ? 63u : 64u * (c.vec_base.sub ? fast_chunks<K, true>() : fast_chunks<K, false>());
+            const size_t waves  = (c.vec_n + (t63 ? 0u : c.lane_shift) + per - 1) / per; // one per wavefront
+            size_t       blocks = (waves + 3) / 4;
             if(blocks == 0)
                 blocks = 1;
             if(blocks > 0x7fffffffu)

@@ -9,6 +9,8 @@
 // head actually consumed).
 #pragma once
 
+#include <type_traits>
+
 #include "rr_launch.hpp"
 
 namespace rr
@@ -213,14 +215,20 @@ __device__ __forceinline__ void generic_vec(const counter_call& c, size_t v, uin
 }
 
 // Fast path for the 32-bit-word engines when every vec shares one engine position
-// modulo the block (no head-shift quirk) and the counter's words above bit 63 do
-// not change within the call: each lane runs exactly one engine block per vec.
-// When the stream starts mid-block (SUB), a wavefront covers 63 vecs with 64
-// consecutive blocks and each lane takes the words it is missing from its
-// neighbour's block (one cross-lane shuffle per word) instead of running a second
-// block.  `first` is the wavefront's first vec; every lane of the wavefront calls this.
+// modulo the block (no head-shift quirk) and the counter's words above bit 63 do not
+// change within the call: each lane runs one engine block per vec.
+//
+// Vecs go in chunks of 64, one per lane, whose starts are congruent to -lane_shift
+// modulo 64, so a full chunk's store covers whole 128-byte lines of the output (a
+// wavefront store that straddles line boundaries measured 31% slower).  This wavefront
+// runs `chunks` consecutive chunks from vec `vb`; lanes outside [lo, end) compute but do
+// not store.  When the stream starts mid-block (SUB) vec v takes words from blocks v and
+// v + 1: a lane holds block v, the next lane block v + 1 (one DPP wavefront shift per
+// word), and lane 63 takes lane 0 of the next chunk's blocks, which are computed one
+// chunk ahead (chunks + 1 block evaluations in all).  Every lane of the wavefront calls
+// this.
 template<ctr_kind K, class D, bool SUB>
-__device__ __forceinline__ void fast_tile(const counter_call& c, size_t first, size_t end)
+__device__ __forceinline__ void fast_run(const counter_call& c, int64_t vb, uint32_t chunks, int64_t lo, int64_t end)
 {
     using T                = ctr_traits<K>;
     using W                = typename T::word;
@@ -239,36 +247,118 @@ __device__ __forceinline__ void fast_tile(const counter_call& c, size_t first, s
     const uint32_t sub  = c.vec_base.sub;
     const uint32_t lane = threadIdx.x & 63u;
     out_t* const   vec_data = static_cast<out_t*>(c.data) + c.mis;
+    // block of vec index b (b outside the call gives words no lane stores)
+    auto block_at = [&](int64_t b, W* w) {
+        const uint64_t cc = clo + (uint64_t)b;
+        W              ctr[DIM];
+        ctr[0] = (W)cc;
+        ctr[1] = (W)(cc >> 32);
+        if constexpr(DIM == 4)
+        {
+            ctr[2] = c2;
+            ctr[3] = c3;
+        }
+        T::block(ctr, c.key, w);
+    };
 
-    const size_t   v  = first + lane;
-    const uint64_t cc = clo + v;
+    W cur[DIM];
+    block_at(vb + lane, cur);
+    for(uint32_t k = 0; k < chunks; k++)
+    {
+        const int64_t base = vb + 64 * (int64_t)k;
+        if(base >= end)
+            break;
+        const int64_t v     = base + lane;
+        const bool    ahead = SUB || k + 1 < chunks;
+        W             nxt[DIM];
+        if(ahead)
+            block_at(v + 64, nxt);
+        W words[DIM];
+        if constexpr(SUB)
+        {
+            W nb[DIM];
+#pragma unroll
+            for(int i = 0; i < DIM; i++)
+            {
+                // wave_shl:1 (lane l reads lane l + 1); lane 63 keeps `old`, the next
+                // chunk's first block word
+                const int first_next = __builtin_amdgcn_readlane((int)nxt[i], 0);
+                nb[i] = (W)__builtin_amdgcn_update_dpp(first_next, (int)cur[i], 0x130, 0xf, 0xf, false);
+            }
+            interleave<W, DIM>(cur, nb, sub, words);
+        }
+        else
+        {
+#pragma unroll
+            for(int i = 0; i < DIM; i++)
+                words[i] = cur[i];
+        }
+        if(v >= lo && v < end)
+        {
+            out_t o[FULL];
+#pragma unroll
+            for(int s = 0; s < OPT; s++)
+                D::apply(words + s * IN, o + s * OUT, c.params);
+            uint32_t u[BYTES / 4];
+            __builtin_memcpy(u, o, BYTES);
+            store_vec<BYTES>(vec_data + v * FULL, u);
+        }
+        if(ahead)
+        {
+#pragma unroll
+            for(int i = 0; i < DIM; i++)
+                cur[i] = nxt[i];
+        }
+    }
+}
+
+// The ALU-bound ThreeFry engines' mid-block start (SUB) without the look-ahead block: a
+// wavefront covers 63 vecs with 64 consecutive blocks (the one-block-per-chunk extra of
+// fast_run costs them more than the aligned stores gain: ThreeFry4x32 uniform f32 at
+// offset 2^20 + 7 measured 3.10 TB/s with it, 3.98 this way).  `first` is the wavefront's
+// first vec; every lane of the wavefront calls this.
+template<ctr_kind K, class D>
+__device__ __forceinline__ void fast_tile63(const counter_call& c, size_t first, size_t end)
+{
+    using T                = ctr_traits<K>;
+    using W                = typename T::word;
+    using out_t            = typename D::out_t;
+    constexpr int DIM      = T::DIM;
+    constexpr int IN       = D::in_w;
+    constexpr int OUT      = D::out_w;
+    constexpr int OPT      = DIM / IN;
+    constexpr int FULL     = OPT * OUT;
+    constexpr int BYTES    = FULL * (int)sizeof(out_t);
+    static_assert(sizeof(W) == 4, "32-bit-word engines only");
+
+    const uint64_t clo  = c.vec_base.c[0] | (c.vec_base.c[1] << 32);
+    const uint32_t lane = threadIdx.x & 63u;
+    out_t* const   vec_data = static_cast<out_t*>(c.data) + c.mis;
+    const size_t   v        = first + lane;
+    const uint64_t cc       = clo + v;
     W              ctr[DIM];
     ctr[0] = (W)cc;
     ctr[1] = (W)(cc >> 32);
     if constexpr(DIM == 4)
     {
-        ctr[2] = c2;
-        ctr[3] = c3;
+        ctr[2] = (W)c.vec_base.c[2];
+        ctr[3] = (W)c.vec_base.c[3];
     }
     W w[DIM];
     T::block(ctr, c.key, w);
-    W words[DIM];
-    if constexpr(SUB)
-    {
-        W nx[DIM];
+    W nx[DIM];
+#pragma unroll
+    for(int i = 0; i < DIM; i++)
+        nx[i] = (W)__builtin_amdgcn_update_dpp(0, (int)w[i], 0x130, 0xf, 0xf, false); // lane l + 1
+    if(v >= end || lane == 63u)
+        return;
+    // the word offset as a constant (a run-time one can leave the eight words in scratch)
+    auto finish = [&](auto sub_c) {
+        constexpr uint32_t S = decltype(sub_c)::value;
+        W                  words[DIM];
 #pragma unroll
         for(int i = 0; i < DIM; i++)
-            nx[i] = __shfl_down(w[i], 1, 64);
-        interleave<W, DIM>(w, nx, sub, words);
-    }
-    else
-    {
-#pragma unroll
-        for(int i = 0; i < DIM; i++)
-            words[i] = w[i];
-    }
-    if(v < end && (!SUB || lane != 63u))
-    {
+            words[i] = (uint32_t)i + S < (uint32_t)DIM ? w[i + S] : nx[i + S - DIM];
         out_t o[FULL];
 #pragma unroll
         for(int s = 0; s < OPT; s++)
@@ -276,10 +366,20 @@ __device__ __forceinline__ void fast_tile(const counter_call& c, size_t first, s
         uint32_t u[BYTES / 4];
         __builtin_memcpy(u, o, BYTES);
         store_vec<BYTES>(vec_data + v * FULL, u);
+    };
+    const uint32_t sub = c.vec_base.sub; // 1 .. DIM - 1
+    if(sub == 1)
+        finish(std::integral_constant<uint32_t, 1>{});
+    else if constexpr(DIM == 4)
+    {
+        if(sub == 2)
+            finish(std::integral_constant<uint32_t, 2>{});
+        else
+            finish(std::integral_constant<uint32_t, 3>{});
     }
 }
 
-// Whether fast_tile applies to a call, and with which wavefront tile.
+// Whether fast_run applies to a call.
 template<ctr_kind K>
 __host__ __device__ inline bool fast_ok(const counter_call& c)
 {

@@ -29,6 +29,7 @@ struct counter_call
     int      quirk;      // vec_base0 != vec_base
     uint32_t ref_stride; // grid stride of the reference's generate kernel for the ordering
                          // (CTR_REF_STRIDE, or the DYNAMIC grid)
+    uint32_t lane_shift; // vecs between the last 128-byte line boundary and vec 0
     void*    data;       // output base pointer
     size_t   n;          // number of outputs
     size_t   mis;        // misalignment (outputs before the first full vec)
