@@ -88,7 +88,7 @@ struct sobol_call
     uint32_t    offset;   // index of the first point
     uint32_t    dims;
     // filled by launch_sobol
-    uint64_t    first_group;
+    int64_t     first_group; // may be -1 (see launch_sobol)
     uint32_t    groups_per_wave;
     uint32_t    waves;
     dist_params params;

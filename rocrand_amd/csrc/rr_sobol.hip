@@ -145,8 +145,18 @@ __global__ __launch_bounds__(256) void sobol_kernel(const sobol_call c)
     if(wave >= c.waves)
         return;
 
-    // L(lane): the low six Gray-code bits
-    const uint32_t gl = lane ^ (lane >> 1);
+    // A wavefront's 64 lanes write 64 consecutive outputs.  To make each such store start
+    // on a 128-byte line (64-byte for 1-byte outputs) the lanes cover points
+    // 64 G + d + lane, d = (offset - output element address) mod the line's element count:
+    // lanes with lane + d < 64 are points of group G, the others of group G + 1.  A point's
+    // value is W(group) ^ L(low six bits): L per lane, W for both groups per wavefront.
+    constexpr uint32_t U   = (sizeof(T) >= 2 ? 128u : 64u) / (uint32_t)sizeof(T);
+    const uint32_t     d   = (uint32_t)((c.offset - reinterpret_cast<uintptr_t>(out) / sizeof(T)) & (U - 1));
+    const uint32_t     low = (lane + d) & 63u;
+    const bool         hi  = lane + d >= 64u;
+
+    // L: the low six Gray-code bits of the lane's point
+    const uint32_t gl = low ^ (low >> 1);
     V              L  = 0;
 #pragma unroll
     for(int k = 0; k < 6; k++)
@@ -154,27 +164,30 @@ __global__ __launch_bounds__(256) void sobol_kernel(const sobol_call c)
             L ^= vec[k];
     L ^= sc;
 
-    // W(A) for the wavefront's first group.  G counts 64-point groups of the output
-    // range; the direction values use the 32-bit point index, i.e. A = G mod 2^26.
-    const uint64_t G  = c.first_group + (uint64_t)wave * c.groups_per_wave;
+    // W(A) for the wavefront's first group.  G counts 64-point groups of the output range
+    // (from one group before the first, for the shift d); the direction values use the
+    // 32-bit point index, i.e. A = G mod 2^26.
+    const int64_t  G  = c.first_group + (int64_t)wave * c.groups_per_wave;
     uint32_t       A  = (uint32_t)G & 0x03ffffffu;
     const uint32_t gA = A ^ (A >> 1);
-    V              W  = (A & 1u) ? vec[5] : V(0);
+    V              Wc = (A & 1u) ? vec[5] : V(0);
     for(int k = 0; k < 26; k++)
         if(gA & (1u << k))
-            W ^= vec[6 + k];
+            Wc ^= vec[6 + k];
+    // W(A + 1): one Gray-code step from W(A)
+    auto step = [&](V w, uint32_t a1) { return a1 == 0 ? V(0) : V(w ^ vec[5] ^ vec[6 + __builtin_ctz(a1)]); };
+    uint32_t An = (A + 1u) & 0x03ffffffu;
+    V        Wn = step(Wc, An);
 
-    int64_t e = (int64_t)(G * 64u + lane) - (int64_t)c.offset;
+    int64_t e = G * 64 + (int64_t)(d + lane) - (int64_t)c.offset;
     for(uint32_t m = 0; m < c.groups_per_wave; m++)
     {
         if(e >= 0 && (uint64_t)e < c.size)
-            store_one(out + e, sob_apply<K, V>(W ^ L, c.params));
+            store_one(out + e, sob_apply<K, V>((hi ? Wn : Wc) ^ L, c.params));
         e += 64;
-        A = (A + 1u) & 0x03ffffffu;
-        if(A == 0)
-            W = 0; // the 32-bit point index wrapped to 0
-        else
-            W ^= vec[5] ^ vec[6 + __builtin_ctz(A)];
+        Wc = Wn;
+        An = (An + 1u) & 0x03ffffffu;
+        Wn = step(Wn, An);
     }
 }
 
@@ -218,10 +231,12 @@ hipError_t launch_sobol(bool is64, out_kind o, sobol_call c, hipStream_t s)
 {
     if(c.size == 0 || c.dims == 0)
         return hipSuccess;
-    // Points [offset, offset + size) of every dimension, in 64-point groups.
+    // Points [offset, offset + size) of every dimension, in 64-point groups, from one
+    // group earlier: the kernel shifts a dimension's groups by up to 63 points to align
+    // its stores.
     const uint64_t first = c.offset / 64u;
     const uint64_t last  = ((uint64_t)c.offset + c.size + 63u) / 64u; // exclusive
-    const uint64_t groups = last - first;
+    const uint64_t groups = last - first + 1;
     // 16 groups (1024 points) per wavefront keeps the uniform set-up negligible
     // while a 2^20-point dimension still spans 256 wavefronts.
     uint32_t gpw = 16;
@@ -230,7 +245,7 @@ hipError_t launch_sobol(bool is64, out_kind o, sobol_call c, hipStream_t s)
     const uint64_t waves = (groups + gpw - 1) / gpw;
     if(waves > 0xffffffffull / 4 || c.dims > 65535)
         return hipErrorInvalidValue;
-    c.first_group     = first;
+    c.first_group     = (int64_t)first - 1;
     c.groups_per_wave = gpw;
     c.waves           = (uint32_t)waves;
     const dim3 grid((uint32_t)((waves + 3) / 4), c.dims);
