@@ -779,6 +779,13 @@ protected:
         c.head   = (uint32_t)std::min<size_t>(n, c.mis);
         c.tail   = (uint32_t)((n - c.head) % s.out_w);
         c.vec_n  = (n - c.head) / s.out_w;
+        {
+            // the kernels shift engines over lanes so that a wavefront's 64 vecs start on a
+            // 128-byte line of the output
+            const uintptr_t vaddr  = addr + (uintptr_t)c.mis * s.esz;
+            const unsigned  vbytes = s.out_w * s.esz;
+            c.lane_shift = 128u % vbytes == 0 ? (uint32_t)((vaddr / vbytes) % (128u / vbytes)) : 0u;
+        }
         c.states     = cur_states();
         c.states_out = m_states + (m_cur ^ 1u) * state_words();
         c.start      = m_start;

@@ -53,6 +53,7 @@ struct inter_call
 {
     uint32_t    base[2][6];
     uint32_t    start;    // offset mod S
+    uint32_t    shift;    // vecs between the last 128-byte line boundary and vec 0
     void*       data;
     size_t      n, mis, vec_n;
     uint32_t    head, tail;
@@ -146,25 +147,29 @@ __global__ __launch_bounds__(256) void batch_inter_kernel(const inter_call* call
     for(uint32_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x)
     {
         const batch_tile tl = tiles[ti];
-        const uint32_t   id = (uint32_t)tl.first + threadIdx.x;
         inter_call       c  = calls[tl.stream];
         stage_tile_table<D>(c.params, staged);
-        if(threadIdx.x >= tl.count)
+        // The tile's engines start `lead` lanes into the block (the stream's first tile
+        // only), so that a wavefront's stores start on a 128-byte line of the output.
+        const uint32_t lead = (uint32_t)((tl.first + c.shift) & 255u);
+        if(threadIdx.x < lead || threadIdx.x - lead >= tl.count)
             continue;
+        const uint32_t id = (uint32_t)tl.first + threadIdx.x - lead;
         // engine (id + start) mod S; the host gave the state of the tile's first engine
         // (first + start, or first + start - S when the whole tile has wrapped), so a lane
-        // is X^j from it, j = its lane index; in the tile that wraps, the wrapped lanes are
-        // X^(j - w) from base[1] (engine 0 with the extra step)
+        // is X^j from it, j = its offset from that engine; in the tile that wraps, the
+        // wrapped lanes are X^(j - w) from base[1] (engine 0 with the extra step)
         const uint32_t e0 = (uint32_t)tl.first + c.start;
+        const uint32_t jl = threadIdx.x - lead; // offset from the tile's first engine
         const uint32_t w  = e0 < S ? S - e0 : 0u;
-        const bool     wr = e0 < S && threadIdx.x >= w;
+        const bool     wr = e0 < S && jl >= w;
         uint32_t       w6[6];
 #pragma unroll
         for(int q = 0; q < 6; q++)
             w6[q] = wr ? c.base[1][q] : tile_states[(size_t)ti * 6 + q];
         ieng<K> g;
         g.set(w6);
-        const uint32_t j = wr ? threadIdx.x - w : threadIdx.x;
+        const uint32_t j = wr ? jl - w : jl;
         if(j != 0u)
             g.sub_entry(t, j);
         out_t* const data     = static_cast<out_t*>(c.data);
@@ -765,6 +770,11 @@ rocrand_status build(batch_layout& L, const rocrand_batch_stream* streams, size_
             c.tail     = (uint32_t)((b.n - c.head) % s.out_w);
             c.vec_n    = (b.n - c.head) / s.out_w;
             c.params   = prm;
+            {
+                const uintptr_t vaddr  = addr + (uintptr_t)c.mis * s.esz;
+                const unsigned  vbytes = s.out_w * s.esz;
+                c.shift = 128u % vbytes == 0 ? (uint32_t)((vaddr / vbytes) % (128u / vbytes)) : 0u;
+            }
             g.icalls.push_back(c);
             const bool   extra = s.out_w > 1 && (c.head > 0 || c.tail > 0);
             const size_t lanes = std::min<size_t>(INTERLEAVED_ENGINES, c.vec_n + (extra ? 1 : 0));
@@ -774,9 +784,13 @@ rocrand_status build(batch_layout& L, const rocrand_batch_stream* streams, size_
             const uint32_t* radix = host_mrg_radix(32);
             uint32_t        x[6];
             bool            have = false, wrapped = false;
-            for(size_t e = 0; e < lanes; e += MRG_TILE)
+            // Tiles on the grid shifted by c.shift: tile t covers engines [256 t - shift,
+            // 256 (t + 1) - shift) of the used ones, its first (active) engine `e`.
+            size_t e = 0, prev = 0;
+            for(size_t t = 0; e < lanes; t++)
             {
-                g.tiles.push_back(batch_tile{si, (uint32_t)std::min<size_t>(MRG_TILE, lanes - e), e});
+                const size_t end = std::min<size_t>((t + 1) * MRG_TILE - c.shift, lanes); // > e
+                g.tiles.push_back(batch_tile{si, (uint32_t)(end - e), e});
                 const uint64_t e0 = e + c.start;
                 if(!have || (!wrapped && e0 >= INTERLEAVED_ENGINES))
                 {
@@ -785,9 +799,11 @@ rocrand_status build(batch_layout& L, const rocrand_batch_stream* streams, size_
                     host_sub_jump(radix, (uint32_t)(wrapped ? e0 - INTERLEAVED_ENGINES : e0), x);
                     have = true;
                 }
-                else
-                    host_entry_apply(radix + 256 * MRG_RADIX_WORDS, x);
+                else // e - prev engines further: 256, or 256 - shift after the first tile
+                    host_entry_apply(radix + (e - prev) * MRG_RADIX_WORDS, x);
                 g.tile_states.insert(g.tile_states.end(), x, x + 6);
+                prev = e;
+                e    = end;
             }
         }
         g.bytes_out += b.n * (size_t)s.esz;

@@ -48,6 +48,7 @@ struct interleaved_call
                             // buffer, so no block of a launch reads a state another block writes)
     uint32_t    start;    // start engine
     uint32_t    S;        // engines (INTERLEAVED_ENGINES, or the DYNAMIC grid), a multiple of 256
+    uint32_t    lane_shift; // vecs between the last 128-byte line boundary and vec 0
     void*       data;
     size_t      n, mis;
     uint32_t    head, tail;
