@@ -58,6 +58,20 @@ size_t poisson_lds_bytes(const interleaved_call& c)
     return dist_kind<D>::value == out_kind::poisson && !c.params.huge ? c.params.pt.size * sizeof(uint64_t) : 0;
 }
 
+// Lane p of an S-lane grid (0 <= p < S) runs engine (p - shift) mod S, so that the
+// wavefront of lanes 64w..64w+63 stores engines 64w - shift onwards, whose first vec sits
+// on a 128-byte line of the output.
+__device__ __forceinline__ uint32_t lane_engine(uint32_t p, uint32_t shift, uint32_t S)
+{
+    return p >= shift ? p - shift : p + S - shift;
+}
+
+// The highest engine of 256-lane block b: the one with the fewest vec-steps.
+__device__ __forceinline__ uint32_t last_engine(uint32_t b, uint32_t shift, uint32_t S)
+{
+    return b == 0 && shift != 0 ? S - 1 : b * 256u + 255u - shift;
+}
+
 template<inter_kind K, class D>
 __global__ __launch_bounds__(256) void generate_kernel(interleaved_call c)
 {
@@ -70,11 +84,9 @@ __global__ __launch_bounds__(256) void generate_kernel(interleaved_call c)
     stage_poisson_table<D>(c);
     const uint32_t S = c.S;
     // engines over lanes shifted by lane_shift, so that each wavefront's stores start on
-    // a 128-byte line (the grid has one more block when the shift is not 0)
-    const int64_t idl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - c.lane_shift;
-    if(idl < 0 || idl >= (int64_t)S)
-        return;
-    const uint32_t id  = (uint32_t)idl;
+    // a 128-byte line; the last lane_shift engines wrap round to block 0's first lanes
+    // (an extra block instead left one CU with three blocks of a two-per-CU grid)
+    const uint32_t id = lane_engine(blockIdx.x * blockDim.x + threadIdx.x, c.lane_shift, S);
     const uint32_t eid = (id + c.start) % S;
     ieng<K>        g;
     g.load(c.states, eid, S);
@@ -97,8 +109,7 @@ __global__ __launch_bounds__(256) void generate_kernel(interleaved_call c)
     // The block's last engine takes the fewest vec-steps: that many run for every lane
     // under a scalar counter, storing from a block-uniform base (buffer stores), and the
     // rest (at most one) per lane.
-    const uint32_t id_last
-        = (uint32_t)std::min<int64_t>((int64_t)blockIdx.x * blockDim.x + blockDim.x - 1 - c.lane_shift, S - 1);
+    const uint32_t id_last = last_engine(blockIdx.x, c.lane_shift, S);
     const uint32_t ucnt = id_last < c.vec_n ? (uint32_t)((c.vec_n - id_last + S - 1) / S) : 0u;
     const uint32_t     off     = id * (uint32_t)BYTES;
     const uint32_t     lim     = S * (uint32_t)BYTES;
@@ -171,18 +182,15 @@ __global__ __launch_bounds__(256) void generate_chunked_kernel(interleaved_call 
     // S is a multiple of the block size: the chunk index is uniform per block, so the
     // jump's digit loop and its table rows stay in scalar registers
     const uint32_t     S   = c.S;
-    const uint32_t     BPC = S / 256 + (c.lane_shift ? 1u : 0u); // blocks per chunk
+    const uint32_t     BPC = S / 256; // blocks per chunk
     const uint32_t     j   = blockIdx.x / BPC;
-    // engines over lanes shifted by lane_shift (see generate_kernel); lanes without an
-    // engine take part in the block's barriers and then leave
-    const int64_t      idl    = (int64_t)(blockIdx.x % BPC) * 256 + threadIdx.x - c.lane_shift;
-    const bool         active = idl >= 0 && idl < (int64_t)S;
-    const uint32_t     id     = active ? (uint32_t)idl : 0u;
+    // engines over lanes shifted by lane_shift, wrapping as in generate_kernel
+    const uint32_t     id  = lane_engine((blockIdx.x % BPC) * 256 + threadIdx.x, c.lane_shift, S);
     if(j >= J)
         return; // the whole block
     stage_poisson_table<D>(c);
     // vec-steps of this engine: vec indices id, id + S, ... below vec_n
-    const uint64_t steps  = active && id < c.vec_n ? (c.vec_n - id + S - 1) / S : 0;
+    const uint64_t steps  = id < c.vec_n ? (c.vec_n - id + S - 1) / S : 0;
     const uint64_t k0     = (uint64_t)j * C;
     const uint64_t end_j  = steps / C < J ? steps / C : J - 1; // chunk that ends at `steps`
     const bool     ending = j == end_j;
@@ -196,12 +204,12 @@ __global__ __launch_bounds__(256) void generate_chunked_kernel(interleaved_call 
         __shared__ uint32_t mat[ieng<K>::LDS_WORDS];
         eng.load(c.states, eid, S);
         eng.advance_block(t, k0 * IN, mat);
-        if(idle || !active)
+        if(idle)
             return;
     }
     else
     {
-        if(idle || !active)
+        if(idle)
             return;
         eng.load(c.states, eid, S);
         eng.advance(t, k0 * IN);
@@ -218,8 +226,7 @@ __global__ __launch_bounds__(256) void generate_chunked_kernel(interleaved_call 
     // more, so that many run under a scalar loop counter and the rest (at most one) per
     // lane.  Stores go to a block-uniform base (scalar registers, advanced by S vecs a
     // step) plus the lane's 32-bit byte offset.
-    const uint32_t id_last
-        = (uint32_t)std::min<int64_t>((int64_t)(blockIdx.x % BPC) * 256 + 255 - c.lane_shift, S - 1);
+    const uint32_t id_last = last_engine(blockIdx.x % BPC, c.lane_shift, S);
     const uint64_t steps_lo = id_last < c.vec_n ? (c.vec_n - id_last + S - 1) / S : 0;
     const uint32_t ucnt
         = steps_lo > k0 ? (uint32_t)(steps_lo - k0 < C ? steps_lo - k0 : (uint64_t)C) : 0u;
@@ -306,7 +313,7 @@ hipError_t launch_gen(const interleaved_call& c, hipStream_t s)
                 hipError_t     err = get_device_tables(t);
                 if(err != hipSuccess)
                     return err;
-                hipLaunchKernelGGL((generate_chunked_kernel<K, D>), dim3((S / 256 + (c.lane_shift ? 1u : 0u)) * J), dim3(256),
+                hipLaunchKernelGGL((generate_chunked_kernel<K, D>), dim3((S / 256) * J), dim3(256),
                                    poisson_lds_bytes<D>(c), s, c, t, J, (uint32_t)C);
                 return hipGetLastError();
             }
@@ -326,12 +333,12 @@ hipError_t launch_gen(const interleaved_call& c, hipStream_t s)
             if(err != hipSuccess)
                 return err;
             const uint32_t C = (uint32_t)((kmax + J - 1) / J);
-            hipLaunchKernelGGL((generate_chunked_kernel<K, D>), dim3((S / 256 + (c.lane_shift ? 1u : 0u)) * J), dim3(256),
+            hipLaunchKernelGGL((generate_chunked_kernel<K, D>), dim3((S / 256) * J), dim3(256),
                                poisson_lds_bytes<D>(c), s, c, t, J, C);
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((generate_kernel<K, D>), dim3(S / 256 + (c.lane_shift ? 1u : 0u)), dim3(256), poisson_lds_bytes<D>(c), s, c);
+    hipLaunchKernelGGL((generate_kernel<K, D>), dim3(S / 256), dim3(256), poisson_lds_bytes<D>(c), s, c);
     return hipGetLastError();
 }
 
