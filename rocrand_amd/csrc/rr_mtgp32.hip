@@ -9,10 +9,22 @@
 // ring, the recurrence parameters and the tempering table of its parameter set;
 // every generator masks with the first set's mask (rocrand_mtgp32.h:381).
 //
-// The ring lives in LDS for the whole call.  One barrier per block-step suffices:
-// a step reads ring[off .. off+350] and writes ring[off+351 .. off+606], so reads
-// and writes of one step never alias, and the next step's reads see this step's
-// writes after the barrier.
+// The ring lives in LDS for the whole call, twice: word i at ring[i] and ring[i + 1024],
+// so a step's reads at off + t + {0, 1, pos - 1, pos} (off < 1024, t < 256, pos <= 95)
+// need no wrap-around masks and share one address (the two pairs differ by immediate
+// offsets from it and from pos - 1).  Every write goes to both copies.  One barrier per
+// block-step suffices: a step reads ring[off .. off+350] and writes ring[off+351 ..
+// off+606] (indices mod 1024), so reads and writes of one step never alias, and the next
+// step's reads see this step's writes after the barrier.
+//
+// The recurrence table of every parameter set is GF(2)-linear in its 4-bit index
+// (tbl[i ^ j] = tbl[i] ^ tbl[j], tbl[0] = 0: MTGP is an F2-linear generator; the host
+// checks it for all 512 sets when it builds the states), so its lookup, which sits on
+// the step's critical path (ring reads -> new word -> ring write -> barrier), is the XOR
+// of the entries at 1, 2, 4 and 8 selected by the index bits: about ten VALU operations
+// on block-uniform operands instead of a dependent LDS round trip.  The tempering
+// table's lookup only feeds the output; it stays an LDS read, issued as soon as the
+// ring reads return so that its latency overlaps the recurrence.
 #include "rr_generator.hpp"
 
 #include <rocrand.h>
@@ -65,6 +77,14 @@ struct mtgp_call
     dist_params params;
 };
 
+// tbl[k & 15] of a GF(2)-linear 16-entry table from its entries at 1, 2, 4 and 8
+__host__ __device__ __forceinline__ uint32_t mtgp_lin16(uint32_t k, uint32_t e1, uint32_t e2, uint32_t e4,
+                                                        uint32_t e8)
+{
+    return (e1 & (0u - (k & 1u))) ^ (e2 & (0u - ((k >> 1) & 1u))) ^ (e4 & (0u - ((k >> 2) & 1u)))
+           ^ (e8 & (0u - ((k >> 3) & 1u)));
+}
+
 template<class D>
 __global__ __launch_bounds__(MTGP_BLOCK) void mtgp_kernel(mtgp_call c)
 {
@@ -74,36 +94,43 @@ __global__ __launch_bounds__(MTGP_BLOCK) void mtgp_kernel(mtgp_call c)
     constexpr int BYTES = OUT * (int)sizeof(out_t);
     static_assert(BYTES == 2 || BYTES == 4 || BYTES == 8 || BYTES == 16, "vec size");
 
-    __shared__ uint32_t ring[MTGP_RING];
-    __shared__ uint32_t ptbl[16], ttbl[16];
+    __shared__ uint32_t ring[2 * MTGP_RING];
+    __shared__ uint32_t ttbl[16];
 
     const uint32_t b = blockIdx.x, t = threadIdx.x;
     mtgp_state*    g = c.states + b;
     for(uint32_t i = t; i < MTGP_RING; i += MTGP_BLOCK)
-        ring[i] = g->status[i];
-    if(t < 16)
     {
-        ptbl[t] = g->tbl[t];
-        ttbl[t] = g->tmp[t];
+        const uint32_t w   = g->status[i];
+        ring[i]             = w;
+        ring[i + MTGP_RING] = w;
     }
     const uint32_t pos = g->pos, sh1 = g->sh1, sh2 = g->sh2, mask = g->mask;
+    if(t < 16)
+        ttbl[t] = g->tmp[t];
+    const uint32_t p1 = g->tbl[1], p2 = g->tbl[2], p4 = g->tbl[4], p8 = g->tbl[8];
     uint32_t       off = g->offset;
     __syncthreads();
 
     // one block-step: this thread's tempered word
     auto step = [&]() -> uint32_t {
-        uint32_t X = (ring[(t + off) & MTGP_MASKI] & mask) ^ ring[(t + off + 1) & MTGP_MASKI];
-        uint32_t Y = ring[(t + off + pos) & MTGP_MASKI];
-        uint32_t T = ring[(t + off + pos - 1) & MTGP_MASKI];
+        const uint32_t* rp = ring + t + off;
+        uint32_t        X  = (rp[0] & mask) ^ rp[1];
+        uint32_t        Y  = rp[pos];
+        uint32_t        T  = rp[pos - 1];
         X ^= X << sh1;
         Y = X ^ (Y >> sh2);
-        const uint32_t r = Y ^ ptbl[Y & 0x0f];
-        ring[(t + off + MTGP_N) & MTGP_MASKI] = r;
         T ^= T >> 16;
         T ^= T >> 8;
+        const uint32_t tt  = ttbl[T & 0x0f];
+        const uint32_t r   = Y ^ mtgp_lin16(Y, p1, p2, p4, p8);
+        uint32_t* const wp = ring + ((t + off + MTGP_N) & MTGP_MASKI);
+        wp[0]              = r;
+        wp[MTGP_RING]      = r;
+        const uint32_t o   = r ^ tt;
         off = (off + MTGP_BLOCK) & MTGP_MASKI;
         __syncthreads();
-        return r ^ ttbl[T & 0x0f];
+        return o;
     };
 
     out_t* const data     = static_cast<out_t*>(c.data);
@@ -229,6 +256,15 @@ public:
         const uint64_t          s64  = m_seed ^ (m_seed >> 32);
         for(unsigned i = 0; i < MTGP_GENS; i++)
         {
+            // the kernel evaluates the recurrence table from its entries at 1, 2, 4 and 8 (the
+            // tempering table is linear too and checked with it), and its
+            // unmasked ring reads reach off + 255 + pos with off < 1024
+            if(p[i].pos == 0 || p[i].pos + MTGP_BLOCK > MTGP_N)
+                return ROCRAND_STATUS_INTERNAL_ERROR;
+            for(uint32_t k = 0; k < 16; k++)
+                if(p[i].tbl[k] != mtgp_lin16(k, p[i].tbl[1], p[i].tbl[2], p[i].tbl[4], p[i].tbl[8])
+                   || p[i].tmp[k] != mtgp_lin16(k, p[i].tmp[1], p[i].tmp[2], p[i].tmp[4], p[i].tmp[8]))
+                    return ROCRAND_STATUS_INTERNAL_ERROR;
             mtgp_state& g = h[i];
             memset(&g, 0, sizeof(g));
             const uint32_t hidden = p[i].tbl[4] ^ (p[i].tbl[8] << 16);
