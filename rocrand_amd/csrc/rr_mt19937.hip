@@ -20,10 +20,11 @@
 //     XOR_{set k} A^k s, whose word i is S[i + k] for the start state's sequence S
 //     extended by 31 words.  Bit-identical to the one-coefficient-per-step evaluation.
 //   * generate: one block per 8 generators keeps their 8 x 624 words in LDS (element-major)
-//     for the whole call; a round twists them in three barrier-separated phases ([0,227)
-//     reads only old words, [227,454) and [454,624) read the words 227 before them, already
-//     new); each wavefront then writes 64 consecutive inputs (8 generators x 8 lanes) per
-//     local word.
+//     for the whole call, in two buffers: a round twists the current words into the other
+//     buffer, each thread following chains of words 227 apart (a word's new value needs
+//     the new value 227 words before it, kept in a register), one barrier per round;
+//     each wavefront then writes 64 consecutive inputs (8 generators x 8 lanes) per local
+//     word while the next round's twist may already write the first buffer.
 #include "rr_generator.hpp"
 
 #include <rocrand.h>
@@ -150,10 +151,12 @@ struct mt_call
 __constant__ unsigned mt_special[8]  = {0, 113, 170, 283, 340, 397, 510, 567};
 __constant__ unsigned mt_offsets[11] = {1, 57, 114, 171, 227, 284, 341, 398, 454, 511, 568};
 
-__device__ __forceinline__ unsigned mt_element(unsigned lane, unsigned local)
+// `local` is wavefront-uniform in the generate kernel (scalar division and table load);
+// `special` = mt_special[lane], loaded once per thread.
+__device__ __forceinline__ unsigned mt_element(unsigned lane, unsigned special, unsigned local)
 {
     if(local == 0)
-        return mt_special[lane];
+        return special;
     const unsigned a = (local - 1) / 7, b = (local - 1) - 7 * a;
     return mt_offsets[a] + 7 * lane + b;
 }
@@ -163,46 +166,46 @@ __device__ __forceinline__ unsigned mt_element(unsigned lane, unsigned local)
 // apart) then fall on 64 distinct banks, and the twist's reads are consecutive.
 __device__ __forceinline__ unsigned lds_at(unsigned k, unsigned g) { return k * GPB + g; }
 
-// One round of the recurrence on GPB generators held in LDS.
-__device__ __forceinline__ void twist_block(uint32_t* st, unsigned t)
+// One round of the recurrence on GPB generators: the round's words in `src`, the next
+// round's written to `dst` (both element-major, so item it = 8k + g is word k of
+// generator g).  Word k of the new round is x'[k] = f(x[k], x[k + 1], m) with
+// m = x[k + 397] for k < 227 and m = x'[k - 227] above, and x[624] read as x'[0].
+// A thread takes whole chains k0, k0 + 227, k0 + 454 (k0 < 227) of one generator: each
+// link's m is the previous link's result, kept in a register, and every other operand
+// is an old word, so no thread waits for another's writes; the one exception, x'[0] for
+// k = 623, the thread of chain 169 computes itself from old words.  Nothing in `src`
+// is written, so a single barrier -- the new words visible to the output reads -- ends
+// the round; `src` is next written by the following round's twist, after every thread
+// has passed this barrier and so finished its reads of it.
+__device__ __forceinline__ void twist_block(const uint32_t* src, uint32_t* dst, unsigned t)
 {
-    constexpr unsigned PER = (GPB * 227 + 255) / 256; // items per thread per phase
-    uint32_t           v[PER];
-    // phase 1: k in [0, 227): old words only
-    // phase 2: k in [227, 454): x[k - 227] is new
-    // phase 3: k in [454, 624): x[k - 227] new, and k = 623 reads the new x[0]
-    const unsigned lo[3] = {0, 227, 454}, len[3] = {227, 227, 170};
-#pragma unroll
-    for(int ph = 0; ph < 3; ph++)
+    constexpr unsigned W = GPB * MT_N, D = GPB * 227, J = GPB * MT_M;
+    for(unsigned it = t; it < D; it += 256)
     {
-        const unsigned items = GPB * len[ph];
-#pragma unroll
-        for(unsigned r = 0; r < PER; r++)
+        const uint32_t a = mt_comp(src[it], src[it + GPB], src[it + J]);
+        dst[it]          = a;
+        const uint32_t b = mt_comp(src[it + D], src[it + D + GPB], a);
+        dst[it + D]      = b;
+        const unsigned k3 = it + 2 * D;
+        if(k3 < W)
         {
-            const unsigned it = t + 256 * r;
-            if(it < items)
+            uint32_t nx;
+            if(k3 + GPB < W)
+                nx = src[k3 + GPB];
+            else
             {
-                const unsigned g = it % GPB, k = lo[ph] + it / GPB;
-                v[r] = mt_comp(st[lds_at(k, g)], st[lds_at(wrap(k + 1), g)], st[lds_at(wrap(k + MT_M), g)]);
+                const unsigned g0 = k3 + GPB - W; // word 0 of this generator
+                nx                = mt_comp(src[g0], src[g0 + GPB], src[g0 + J]);
             }
+            dst[k3] = mt_comp(src[k3], nx, b);
         }
-        __syncthreads();
-#pragma unroll
-        for(unsigned r = 0; r < PER; r++)
-        {
-            const unsigned it = t + 256 * r;
-            if(it < items)
-            {
-                const unsigned g = it % GPB, k = lo[ph] + it / GPB;
-                st[lds_at(k, g)] = v[r];
-            }
-        }
-        __syncthreads();
     }
+    __syncthreads();
 }
 
+// 4 blocks per CU (the LDS of 2 x 8 generators each fills it): at most 128 VGPRs
 template<class D>
-__global__ __launch_bounds__(256) void mt_generate_kernel(mt_call c)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void mt_generate_kernel(mt_call c)
 {
     using out_t          = typename D::out_t;
     constexpr int IN     = D::in_w;
@@ -212,36 +215,101 @@ __global__ __launch_bounds__(256) void mt_generate_kernel(mt_call c)
     constexpr uint64_t FULL = (uint64_t)MT_STRIDE * IPS;
     static_assert(BYTES == 4 || BYTES == 8 || BYTES == 16, "vec size");
 
-    __shared__ uint32_t st[GPB * MT_N];
+    // the round's words and the next round's (twist_block), 2 x 39 KiB: 4 blocks per CU
+    __shared__ uint32_t buf[2][GPB * MT_N];
     const unsigned t = threadIdx.x, blk = blockIdx.x;
     uint32_t*      gstate = c.states + (size_t)blk * GPB * MT_N;
+    uint32_t*      st     = buf[0];
+    uint32_t*      nx     = buf[1];
     for(unsigned i = t; i < GPB * MT_N; i += 256)
         st[lds_at(i % MT_N, i / MT_N)] = gstate[i];
     __syncthreads();
 
-    const unsigned p = t & 63u, quarter = t >> 6;
+    // the wavefront index as a scalar: the local word j, its element offsets and the
+    // wavefront's input range below are then uniform (scalar loads and branches)
+    const unsigned p = t & 63u, quarter = __builtin_amdgcn_readfirstlane(t >> 6);
     const unsigned gl = p >> 3, lane = p & 7u;
-    const unsigned tid = blk * 64u + p; // the reference's thread index
     const uint64_t total = c.vec_n + c.extra;
     const uint64_t qend  = (uint64_t)c.start + total;
     out_t* const   data  = static_cast<out_t*>(c.data);
     out_t* const   vec_data = data + c.mis;
+    const unsigned special = mt_special[lane];
     alignas(16) out_t o[OUT];
     uint32_t     in[IN];
+
+    // LDS indices of this thread's inputs, the same every round: local word
+    // j = quarter + 4m (m < MW) reads eoff[m][i] for input i (held in registers: every
+    // loop over m below is unrolled)
+    constexpr unsigned MW = (IPS + 3) / 4;
+    unsigned           eoff[MW][IN];
+#pragma unroll
+    for(unsigned m = 0; m < MW; m++)
+#pragma unroll
+        for(int i = 0; i < IN; i++)
+        {
+            const unsigned j = quarter + 4 * m;
+            eoff[m][i]       = j < IPS ? lds_at(mt_element(lane, special, j * IN + i), gl) : 0;
+        }
 
     for(uint64_t R = 0; R * FULL < qend; R++)
     {
         if(R > 0 || c.start == 0)
-            twist_block(st, t);
+        {
+            twist_block(st, nx, t);
+            uint32_t* const w = st;
+            st                = nx;
+            nx                = w;
+        }
+        const uint64_t R0 = R * FULL;
+        if(R0 >= c.start && R0 + FULL <= c.start + c.vec_n)
+        {
+            // every input of the round is a whole vec: no range checks.  Lane p is the
+            // reference's thread blk * 64 + p; its local word j is input
+            // R0 + j * 65536 + blk * 64 + p.
+            // Up to 8-byte vecs all of the thread's words are read first (an invalid last j
+            // reads word 0 and is not stored), so the LDS reads overlap one another; the
+            // double-pair transforms need those registers themselves.
+            constexpr bool BATCH = BYTES <= 8;
+            out_t* const   rbase = vec_data + (R0 - c.start + blk * 64u + p) * OUT;
+            uint32_t       w[BATCH ? MW : 1][IN];
+            if constexpr(BATCH)
+            {
+#pragma unroll
+                for(unsigned m = 0; m < MW; m++)
+#pragma unroll
+                    for(int i = 0; i < IN; i++)
+                        w[m][i] = st[eoff[m][i]];
+            }
+#pragma unroll
+            for(unsigned m = 0; m < MW; m++)
+            {
+                const unsigned j = quarter + 4 * m;
+                if(j < IPS)
+                {
+#pragma unroll
+                    for(int i = 0; i < IN; i++)
+                        in[i] = mt_temper(BATCH ? w[BATCH ? m : 0][i] : st[eoff[m][i]]);
+                    D::apply(in, o, c.params);
+                    out_t* dst = rbase + (size_t)j * MT_STRIDE * OUT;
+                    if constexpr(BYTES == 4)
+                        *reinterpret_cast<uint32_t*>(dst) = *reinterpret_cast<const uint32_t*>(o);
+                    else if constexpr(BYTES == 8)
+                        *reinterpret_cast<uint2*>(dst) = *reinterpret_cast<const uint2*>(o);
+                    else
+                        *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(o);
+                }
+            }
+            continue;
+        }
         for(unsigned j = quarter; j < IPS; j += 4)
         {
-            const uint64_t Q = R * FULL + (uint64_t)j * MT_STRIDE + tid;
+            const uint64_t Q = R0 + (uint64_t)j * MT_STRIDE + blk * 64u + p;
             if(Q < c.start || Q >= qend)
                 continue;
             const uint64_t v = Q - c.start;
 #pragma unroll
             for(int i = 0; i < IN; i++)
-                in[i] = mt_temper(st[lds_at(mt_element(lane, j * IN + i), gl)]);
+                in[i] = mt_temper(st[lds_at(mt_element(lane, special, j * IN + i), gl)]);
             D::apply(in, o, c.params);
             if(v < c.vec_n)
             {
@@ -264,7 +332,7 @@ __global__ __launch_bounds__(256) void mt_generate_kernel(mt_call c)
                 }
             }
         }
-        __syncthreads();
+        // no barrier: the next round twists into the buffer these reads do not touch
     }
     for(unsigned i = t; i < GPB * MT_N; i += 256)
         gstate[i] = st[lds_at(i % MT_N, i / MT_N)];
