@@ -141,23 +141,28 @@ __global__ __launch_bounds__(MTGP_BLOCK) void mtgp_kernel(mtgp_call c)
     const size_t up       = rem == 0 ? down : down + MTGP_BLOCK;
     alignas(16) out_t o[OUT];
     uint32_t     in[IN];
-    size_t       index = (size_t)b * MTGP_BLOCK + t;
-    // the loop bound is block-uniform: every thread takes part in every step
-    for(; index - t < down; index += stride)
+    // the block's vec index before this thread's, block-uniform (a scalar loop counter and
+    // store base): every thread takes part in every step
+    const size_t   base0 = (size_t)b * MTGP_BLOCK;
+    const uint32_t steps = base0 < down ? (uint32_t)((down - base0 + stride - 1) / stride) : 0u;
+    out_t*         bptr  = vec_data + base0 * OUT;
+    for(uint32_t k = 0; k < steps; k++, bptr += stride * OUT)
     {
 #pragma unroll
         for(int i = 0; i < IN; i++)
             in[i] = step();
         D::apply(in, o, c.params);
+        out_t* const dst = bptr + t * OUT;
         if constexpr(BYTES == 2)
-            *reinterpret_cast<uint16_t*>(vec_data + index * OUT) = *reinterpret_cast<const uint16_t*>(o);
+            *reinterpret_cast<uint16_t*>(dst) = *reinterpret_cast<const uint16_t*>(o);
         else if constexpr(BYTES == 4)
-            *reinterpret_cast<uint32_t*>(vec_data + index * OUT) = *reinterpret_cast<const uint32_t*>(o);
+            *reinterpret_cast<uint32_t*>(dst) = *reinterpret_cast<const uint32_t*>(o);
         else if constexpr(BYTES == 8)
-            *reinterpret_cast<uint2*>(vec_data + index * OUT) = *reinterpret_cast<const uint2*>(o);
+            *reinterpret_cast<uint2*>(dst) = *reinterpret_cast<const uint2*>(o);
         else
-            *reinterpret_cast<uint4*>(vec_data + index * OUT) = *reinterpret_cast<const uint4*>(o);
+            *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(o);
     }
+    size_t index = base0 + (size_t)steps * stride + t;
     if(index - t < up)
     {
 #pragma unroll
