@@ -140,8 +140,11 @@ __global__ __launch_bounds__(256) void sobol_kernel(const sobol_call c)
         }
     }
 
+    // the wavefront index as a scalar, so that the group index below, its Gray-code
+    // steps and their direction-vector loads are scalar too (a per-lane load here would
+    // make every step wait for the wavefront's outstanding stores)
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4u + (threadIdx.x >> 6));
     if(wave >= c.waves)
         return;
 
@@ -174,8 +177,12 @@ __global__ __launch_bounds__(256) void sobol_kernel(const sobol_call c)
     for(int k = 0; k < 26; k++)
         if(gA & (1u << k))
             Wc ^= vec[6 + k];
-    // W(A + 1): one Gray-code step from W(A)
-    auto step = [&](V w, uint32_t a1) { return a1 == 0 ? V(0) : V(w ^ vec[5] ^ vec[6 + __builtin_ctz(a1)]); };
+    // W(A + 1): one Gray-code step from W(A).  The direction vectors are read-only for
+    // the kernel: through the constant address space a wavefront-uniform index becomes a
+    // scalar load, which the stores do not delay.
+    using cvec_t = const __attribute__((address_space(4))) V*;
+    const cvec_t cvec = (cvec_t)vec;
+    auto step = [&](V w, uint32_t a1) { return a1 == 0 ? V(0) : V(w ^ cvec[5] ^ cvec[6 + __builtin_ctz(a1)]); };
     uint32_t An = (A + 1u) & 0x03ffffffu;
     V        Wn = step(Wc, An);
 
