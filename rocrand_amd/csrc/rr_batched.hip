@@ -84,6 +84,25 @@ __device__ __forceinline__ void stage_tile_table(dist_params& p, const uint64_t*
     }
 }
 
+// The plan's descriptors (tiles, calls, tile states) are read-only for the kernels.  Read
+// through the constant address space, a tile's descriptors -- block-uniform -- come in
+// with scalar loads into SGPRs; as vector loads they landed in VGPRs and every tile's
+// first load waited for all the stores of the block's previous tile (vector loads and
+// stores retire in order on one counter).
+template<class T>
+__device__ __forceinline__ T read_const(const T* p, size_t i)
+{
+    static_assert(sizeof(T) % 4 == 0 && alignof(T) >= 4, "dword-copyable descriptor");
+    using cword = const __attribute__((address_space(4))) uint32_t*;
+    const cword q = (cword)(p + i);
+    T           r;
+    uint32_t*   d = reinterpret_cast<uint32_t*>(&r);
+#pragma unroll
+    for(size_t k = 0; k < sizeof(T) / 4; k++)
+        d[k] = q[k];
+    return r;
+}
+
 template<ctr_kind K, class D>
 __global__ __launch_bounds__(256) void batch_counter_kernel(const counter_call* calls,
                                                             const batch_tile*   tiles,
@@ -93,9 +112,11 @@ __global__ __launch_bounds__(256) void batch_counter_kernel(const counter_call* 
     const uint64_t* staged = nullptr;
     for(uint32_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x)
     {
-        const batch_tile tl    = tiles[ti];
-        counter_call     c     = calls[tl.stream];
-        const uint64_t   rmask = (uint64_t)calls[tl.stream].ref_stride - 1;
+        const batch_tile tl = read_const(tiles, ti);
+        // (the Poisson kernel keeps a vector copy: with the descriptor in SGPRs its
+        // positions went to scratch)
+        counter_call c = dist_kind<D>::value == out_kind::poisson ? calls[tl.stream] : read_const(calls, tl.stream);
+        const uint64_t rmask = (uint64_t)c.ref_stride - 1;
         stage_tile_table<D>(c.params, staged);
         const size_t end = tl.first + tl.count;
         if(fast_ok<K>(c))
@@ -105,7 +126,7 @@ __global__ __launch_bounds__(256) void batch_counter_kernel(const counter_call* 
             const int64_t  lo   = (int64_t)tl.first, hi = (int64_t)end;
             const int64_t  vb0  = lo - ((lo + c.lane_shift) & 63);
             const uint32_t nch  = (uint32_t)((hi - vb0 + 63) / 64);
-            const uint32_t wave = threadIdx.x >> 6;
+            const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
             if(c.vec_base.sub)
             {
                 // consecutive chunks per wavefront, so that each run's look-ahead block
@@ -146,8 +167,8 @@ __global__ __launch_bounds__(256) void batch_inter_kernel(const inter_call* call
     const uint64_t* staged = nullptr;
     for(uint32_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x)
     {
-        const batch_tile tl = tiles[ti];
-        inter_call       c  = calls[tl.stream];
+        const batch_tile tl = read_const(tiles, ti);
+        inter_call       c  = read_const(calls, tl.stream);
         stage_tile_table<D>(c.params, staged);
         // The tile's engines start `lead` lanes into the block (the stream's first tile
         // only), so that a wavefront's stores start on a 128-byte line of the output.
@@ -166,7 +187,7 @@ __global__ __launch_bounds__(256) void batch_inter_kernel(const inter_call* call
         uint32_t       w6[6];
 #pragma unroll
         for(int q = 0; q < 6; q++)
-            w6[q] = wr ? c.base[1][q] : tile_states[(size_t)ti * 6 + q];
+            w6[q] = wr ? c.base[1][q] : read_const(tile_states, (size_t)ti * 6 + q);
         ieng<K> g;
         g.set(w6);
         const uint32_t j = wr ? jl - w : jl;
