@@ -82,6 +82,27 @@ __host__ __device__ __forceinline__ W rotl(W x, int d)
     constexpr int B = sizeof(W) * 8;
     return (x << (d & (B - 1))) | (x >> ((B - d) & (B - 1)));
 }
+// A 64-bit rotation as two 32-bit funnel shifts of its halves (one v_alignbit_b32 each
+// for the constant amounts of the rounds, where the 64-bit form takes two 64-bit shifts
+// and two ORs); the same value as the generic form.
+template<>
+__host__ __device__ __forceinline__ uint64_t rotl<uint64_t>(uint64_t x, int d)
+{
+    d &= 63;
+    uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
+    if(d >= 32)
+    {
+        const uint32_t t = hi;
+        hi               = lo;
+        lo               = t;
+        d -= 32;
+    }
+    if(d == 0)
+        return ((uint64_t)hi << 32) | lo;
+    const uint32_t nh = (hi << d) | (lo >> (32 - d));
+    const uint32_t nl = (lo << d) | (hi >> (32 - d));
+    return ((uint64_t)nh << 32) | nl;
+}
 
 template<class W>
 __host__ __device__ constexpr W ks_parity()
