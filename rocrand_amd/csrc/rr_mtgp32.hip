@@ -108,6 +108,16 @@ __global__ __launch_bounds__(MTGP_BLOCK) void mtgp_kernel(mtgp_call c)
     const uint32_t pos = g->pos, sh1 = g->sh1, sh2 = g->sh2, mask = g->mask;
     if(t < 16)
         ttbl[t] = g->tmp[t];
+    // Poisson: the block samples from its copy of the packed alias table in dynamic LDS
+    // (a per-lane load from global memory would wait for the block's earlier stores)
+    if constexpr(dist_kind<D>::value == out_kind::poisson)
+    {
+        extern __shared__ uint64_t lds_packed[];
+        const uint32_t             size = c.params.huge ? 0u : c.params.pt.size;
+        for(uint32_t i = t; i < size; i += MTGP_BLOCK)
+            lds_packed[i] = c.params.pt.packed[i];
+        c.params.pt.packed = lds_packed;
+    }
     const uint32_t p1 = g->tbl[1], p2 = g->tbl[2], p4 = g->tbl[4], p8 = g->tbl[8];
     uint32_t       off = g->offset;
     __syncthreads();
@@ -199,7 +209,10 @@ __global__ __launch_bounds__(MTGP_BLOCK) void mtgp_kernel(mtgp_call c)
 template<class D>
 hipError_t launch_gen(const mtgp_call& c, hipStream_t s)
 {
-    hipLaunchKernelGGL((mtgp_kernel<D>), dim3(MTGP_GENS), dim3(MTGP_BLOCK), 0, s, c);
+    const size_t lds = dist_kind<D>::value == out_kind::poisson && !c.params.huge
+                           ? c.params.pt.size * sizeof(uint64_t)
+                           : 0;
+    hipLaunchKernelGGL((mtgp_kernel<D>), dim3(MTGP_GENS), dim3(MTGP_BLOCK), lds, s, c);
     return hipGetLastError();
 }
 
