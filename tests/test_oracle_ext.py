@@ -315,3 +315,23 @@ def test_mt19937_oracle_equals_reference_kernel_rounds():
         s2, y = rg.generate(kind, n, 0x10000, a, b)
         assert s1 == s2 == 0
         assert _same(kind, x, y), (kind, n)
+
+
+def test_mtgp32_parameter_tables_are_linear():
+    """The MTGP32 kernel evaluates each set's recurrence table from its entries at 1, 2,
+    4 and 8 (an XOR selected by the index bits) and reads the ring at off + t + pos with
+    t < 256 without wrap masks: both rest on properties of the 512 parameter sets
+    (rocrand_mtgp32_11213.h), checked here on the library's data file."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    p = np.fromfile(os.path.join(os.path.dirname(here), "rocrand_amd", "data", "mtgp32_11213.bin"),
+                    dtype=np.uint32).reshape(512, 36)
+    pos, tbl, tmp = p[:, 0], p[:, 3:19], p[:, 19:35]
+    for t in (tbl, tmp):
+        assert not t[:, 0].any()
+        for k in range(16):
+            want = np.zeros(512, dtype=np.uint32)
+            for b in range(4):
+                if k >> b & 1:
+                    want ^= t[:, 1 << b]
+            assert np.array_equal(t[:, k], want), k
+    assert (pos > 0).all() and (pos + 256 <= 351).all()
